@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the tinyMD per-timestep path on one MI355X.
+
+Workload: Lennard-Jones FCC lattice of 80x80x80 unit cells (2,048,000 atoms,
+density 0.8442, lattice constant 1.6796), Park-Miller velocities rescaled to
+T = 1.44, eps = sigma = 1, cutoff 2.5, skin 0.3, dt 0.005, reneighbour every
+20 steps, full neighbour list.  A step is one full timestep (integrate,
+ghost refresh or rebuild, pair force, integrate) over all atoms, with all
+state resident in HBM.
+
+Printed value: atom-steps/s of the FP32-mixed path (FP32 pair forces with
+compensated accumulation, FP64 positions/velocities/ghosts/reductions);
+the FP64 path, the half-list variants and the FP32-vs-FP64 force error are
+reported beside it.  `roofline` is for the dominant kernel (the pair-force
+kernel), timed with HIP events on the library's stream; `cpu_baseline` is
+the FP64 CPU oracle on the host cores over a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "LJ atom-steps/s, FP32-mixed vs FP64 at ≤1e-5 force err; adjoint/fwd cost"
+NC = 80
+RC, SKIN, DT, REN = 2.5, 0.3, 0.005, 20
+
+
+def force_kernel_bytes(sim, precision):
+    """Algorithmic HBM bytes of one pair-force launch (full list): neighbour
+    counts and work indices (4 B each per atom), 4 B per list entry, each
+    position read once (24 B, x/y/z FP64), each force written once (24 B)."""
+    counts, _ = sim.neighbor_lists(with_lists=False)
+    n = sim.nlocal
+    nall = n + sim.nghost
+    entries = int(counts.astype(np.int64).sum())
+    return 4 * n + 4 * n + 4 * entries + 24 * nall + 24 * n, entries
+
+
+def run_variant(tm, precision, half, steps, warmup, profile=False):
+    sim = tm.Simulation.fcc(NC, precision=precision)
+    sim.setup(half, RC, SKIN)
+    sim.lennard_jones(half, RC)
+    if warmup:
+        sim.run_lj(warmup, REN, DT, half, RC, SKIN)
+    sim.barrier()
+    if profile:
+        sim.profile(True)
+        sim.profile_reset()
+    t0 = time.perf_counter()
+    done = sim.run_lj(steps, REN, DT, half, RC, SKIN)
+    sim.barrier()
+    t1 = time.perf_counter()
+    assert done == steps
+    out = {"seconds": t1 - t0, "atom_steps_per_s": sim.nlocal * steps / (t1 - t0)}
+    if profile:
+        ms, cnt = sim.profile_get("force")
+        out["force_ms_avg"] = ms / max(cnt, 1)
+        out["force_launches"] = cnt
+        for k in ("integrate1", "integrate2", "ghost_sync"):
+            kms, kc = sim.profile_get(k)
+            out[k + "_ms_avg"] = kms / max(kc, 1)
+        sim.profile(False)
+    return sim, out
+
+
+def fp32_force_error(tm, sim):
+    """Force error of the FP32-mixed kernel against the FP64 kernel on the
+    same (thermalised) positions and neighbour list."""
+    sim.set_precision("fp64")
+    sim.lennard_jones(False, RC)
+    f64 = sim.state()[3]
+    sim.set_precision("fp32mixed")
+    sim.lennard_jones(False, RC)
+    f32 = sim.state()[3]
+    rms = np.sqrt(np.mean(np.sum(f64 * f64, axis=1)))
+    d = f32 - f64
+    return float(np.sqrt(np.mean(np.sum(d * d, axis=1))) / rms), float(np.max(np.abs(d)) / rms)
+
+
+def cpu_baseline(sample_steps=20):
+    """FP64 oracle (test infrastructure, the checker) on the host cores: the
+    same 80^3 workload, `sample_steps` steps including one rebuild."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = min(os.cpu_count() or 1, 16)
+    s = po.lj_benchmark(NC, threads=threads)
+    t0 = time.perf_counter()
+    s.run_lj(sample_steps, REN, DT)
+    t1 = time.perf_counter()
+    n = s.n
+    s.close()
+    return {"value": n * sample_steps / (t1 - t0), "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, FCC 80^3 ({n} atoms), {sample_steps} steps incl. 1 rebuild, "
+                      f"{threads} OpenMP threads (list build and pair force threaded), {t1 - t0:.1f} s"}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "force_lj_fp32mixed_80cube_traffic.json")
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--quick", action="store_true", help="headline variant only")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU decomposition is design-only "
+              "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import tinymd_amd as tm
+    dev, cus = tm.device_info()
+    if cus == 0:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    print(f"# device: {dev}, {cus} CUs", file=sys.stderr, flush=True)
+
+    variants = {}
+    sim32, r32 = run_variant(tm, "fp32mixed", False, args.steps, args.warmup, profile=True)
+    variants["fp32mixed_full"] = r32
+    print(f"# fp32mixed full: {r32}", file=sys.stderr, flush=True)
+    nbytes, entries = force_kernel_bytes(sim32, "fp32mixed")
+    err_rms, err_max = fp32_force_error(tm, sim32)
+    n = sim32.nlocal
+    sim32.close()
+    del sim32
+    if not args.quick:
+        for prec, half in (("fp64", False), ("fp32mixed", True), ("fp64", True)):
+            sim, r = run_variant(tm, prec, half, args.steps, args.warmup, profile=True)
+            variants[f"{prec}_{'half' if half else 'full'}"] = r
+            print(f"# {prec} half={half}: {r}", file=sys.stderr, flush=True)
+            sim.close()
+            del sim
+
+    value = r32["atom_steps_per_s"]
+    t_force = r32["force_ms_avg"] * 1e-3
+    achieved = nbytes / t_force / 1e9
+    traffic = load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["seconds"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair forces (compensated), f64 state",
+        "data": "synthetic: FCC lattice + Park-Miller velocities (initialize.h), T=1.44",
+        "config": {"workload": "LJ FCC 80x80x80 cells, 2,048,000 atoms, rho 0.8442, rc 2.5, skin 0.3, "
+                               "dt 0.005, reneighbour 20, full list, FP32-mixed",
+                   "atoms": n, "list_entries": entries},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_lj_full_f32", "bytes_per_launch": nbytes,
+                     "avg_launch_ms": r32["force_ms_avg"]},
+        "fp32_vs_fp64_force_err": {"rms_rel": err_rms, "max_abs_over_rms": err_max},
+        "pair_interactions_per_s": None,
+        "variants": variants,
+    }
+    if "fp64_full" in variants:
+        line["fp32mixed_over_fp64"] = value / variants["fp64_full"]["atom_steps_per_s"]
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

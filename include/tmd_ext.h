@@ -1,0 +1,94 @@
+/* Extensions of libtinymd_amd.so beyond the reference ABI (md_abi.h).
+ * None of these has a counterpart in the reference (anydsl_includes.h);
+ * each comment names the reference behaviour it extends. */
+#ifndef TINYMD_AMD_TMD_EXT_H
+#define TINYMD_AMD_TMD_EXT_H
+
+#include "md_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors ------------------------------------------------------------- */
+/* First recorded error (HIP failure, invalid geometry, escaped atoms,
+ * fixed-point overflow), or NULL.  The reference has no error channel. */
+const char* tmd_last_error(void);
+void tmd_clear_error(void);
+/* Device name into buf; returns the number of compute units (0 = no GPU). */
+int tmd_device_info(char* buf, int len);
+
+/* ---- modes ---------------------------------------------------------------- */
+/* 0 = FP64 (default; the reference's real_t, utils/utilities.impala:1),
+ * 1 = FP32-mixed pair forces with compensated accumulation (FP64 positions,
+ *     velocities, wrap, ghosts and reductions). */
+void tmd_set_precision(int mode);
+/* Half-list filter.  0 = local j < i plus every ghost (each periodic pair
+ * once; default), 1 = the reference's literal i > j filter, which drops all
+ * ghost pairs (core/neighborlist.impala:34). */
+void tmd_set_half_mode(int mode);
+/* Ghost refresh.  0 = fresh (owner + image * L; default), 1 = the
+ * reference's two-phase refresh whose ghost-of-ghost values lag a step
+ * (comm/comm.impala:341-387). */
+void tmd_set_ghost_mode(int mode);
+/* 1 (default): every md_* call returns after the GPU finished it.
+ * 0: calls only enqueue; md_barrier / copies synchronise. */
+void tmd_set_sync(int on);
+/* DEM sphere radius; <= 0 restores the default (contact cutoff / 2).
+ * The reference hard-codes 1.0 (core/runtime.impala:210). */
+void tmd_set_dem_radius(double radius);
+/* 1 = shifted-force LJ, F(r) - F(rc), with energy shifted to match
+ * (continuous force at rc; used for the differentiable runs). */
+void tmd_set_lj_shift(int on);
+
+/* ---- thermo (deterministic device reductions) ------------------------------
+ * out = {T, KE, PE, W, P, pairs}: T = sum v^2 / (3N-3) (core/thermo.impala:1-16),
+ * PE and the pair virial W = sum r.F over the current list (the reference
+ * computes neither), P = (sum v^2 + W) / (3V), pairs = in-cutoff entries. */
+void tmd_thermo(double rc, double epsilon, double sigma, double* out6);
+
+/* ---- introspection for parity tests (host copies) --------------------------- */
+/* {nx, ny, nz, ncells, nlocal, nghost, list capacity} */
+void tmd_get_grid_info(int* out7);
+/* Ghost g (slot nlocal + g): flattened owner, total image (3 ints),
+ * immediate source and per-direction flag (3 ints).  Returns nghost. */
+int tmd_get_ghost_table(int* owner, int* image3, int* source, int* flag3);
+/* Cell CSR: start[ncells + 1], atoms[start[ncells]], cell of each atom
+ * [nlocal + nghost] (-1 outside).  Returns the number of binned atoms. */
+int tmd_get_cells(int* start, int* atoms, int* atom_cell);
+/* Neighbour lists by atom index: counts[nlocal], list[nlocal * stride]
+ * (row-major, first min(count, stride) entries).  Returns nlocal. */
+int tmd_get_neighbor_lists(int* counts, int* list, int stride);
+
+/* ---- device-resident step loops -------------------------------------------
+ * main.cpp's timestep loop (main.cpp:715-776) run inside the library with
+ * host synchronisation only at rebuilds.  Return the number of steps run
+ * (0 on error). */
+int tmd_run_lj(int steps, int reneigh_every, double dt, bool half, double cutoff, double skin, double epsilon, double sigma);
+int tmd_run_dem(int steps, int reneigh_every, double dt, bool half, double cutoff, double skin, double damping_n,
+                double damping_t, double stiffness, double friction);
+
+/* ---- profiling: HIP events around each launch on the library stream -------- */
+/* kernel ids: 0 force, 1 initial integration, 2 final integration,
+ * 3 ghost refresh, 4 neighbour list, 5 binning, 6 borders, 7 thermo, 8 adjoint */
+void tmd_profile_enable(int on);
+void tmd_profile_get(int kernel, double* total_ms, long long* count);
+void tmd_profile_reset(void);
+
+/* ---- initial conditions (host) ----------------------------------------------
+ * FCC lattice exactly as init_rectangular_grid builds it (initialize.h:91-224):
+ * returns the number of atoms (arrays filled up to cap). */
+int tmd_generate_fcc(int nx, int ny, int nz, double potential_minimum, int half, double* mass, struct Vector3D* pos,
+                     struct Vector3D* vel, int cap);
+/* Granular gas: FCC sites of an nc^3-cell lattice kept where the lattice
+ * indices satisfy i + j < 2 nc (the diagonal half x + y < L), velocities
+ * uniform in [-vmax, vmax] from a counter-based hash of (seed, site,
+ * component).  Replaces initialize.h:226-295 for the DEM benchmark. */
+int tmd_generate_granular_gas(int nc, double potential_minimum, unsigned long long seed, double vmax, double* mass,
+                              struct Vector3D* pos, struct Vector3D* vel, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TINYMD_AMD_TMD_EXT_H */

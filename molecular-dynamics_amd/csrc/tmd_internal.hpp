@@ -1,0 +1,158 @@
+// Internal declarations shared by the HIP translation units of libtinymd_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace tmd {
+
+// ---------------------------------------------------------------------------
+// error handling: every HIP call is checked; the first failure is recorded,
+// printed, and reported by tmd_last_error(); later work is skipped.
+void set_error(const char* file, int line, const char* what);
+bool has_error();
+
+#define TMD_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::tmd::set_error(__FILE__, __LINE__, hipGetErrorString(e_));       \
+        }                                                                      \
+    } while (0)
+
+#define TMD_REQUIRE(cond, msg)                                                 \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::tmd::set_error(__FILE__, __LINE__, msg);                         \
+        }                                                                      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// modes
+enum Precision { PREC_FP64 = 0, PREC_FP32_MIXED = 1 };
+enum HalfMode { HALF_GHOST_FULL = 0, HALF_REF_LITERAL = 1 };
+enum GhostMode { GHOST_FRESH = 0, GHOST_REF_LAG = 1 };
+
+// Fixed-point scale of the half-list force accumulators (2^40): forces up to
+// 8.4e6 in magnitude, resolution 9.1e-13.
+constexpr double kFixScale = 1099511627776.0;
+constexpr double kFixInv = 1.0 / 1099511627776.0;
+constexpr double kFixMax = 4.0e6;
+
+// Geometry of one periodic box (single rank: rank AABB == world AABB).
+struct Geom {
+    double world[6];  // xmin xmax ymin ymax zmin zmax
+    double aabb[6];   // rank AABB extended by one cell spacing
+    double len[3];    // periodic lengths
+    double spacing;   // cell spacing
+    int nx, ny, nz, ncells;
+};
+
+// A device buffer that grows (optionally preserving its contents).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t need, bool preserve, hipStream_t s);
+    void release();
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Per-kernel timing with HIP events on the library stream.
+enum KernelId {
+    K_FORCE = 0, K_INTEGRATE_1, K_INTEGRATE_2, K_GHOST_SYNC, K_NEIGHBOR, K_BIN, K_BORDERS,
+    K_THERMO, K_ADJOINT, K_NKERNELS
+};
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    double total_ms[K_NKERNELS] = {0};
+    long long count[K_NKERNELS] = {0};
+    size_t next = 0;
+    void begin(int id, hipStream_t s, hipEvent_t* ev0);
+    void end(int id, hipStream_t s, hipEvent_t ev0);
+    void collect();
+    void reset();
+};
+
+struct Sim {
+    Geom g;
+    int n = 0;          // local atoms
+    int ng = 0;         // ghosts
+    int cap_all = 0;    // capacity of per-atom arrays (locals + ghosts)
+    int nb_cap = 100;   // neighbour-list capacity per atom
+    int nb_stride = 0;  // column stride of the list (multiple of 64 >= n)
+    int nb_half = 0;    // list built as a half list
+    int nb_valid = 0;   // list present
+
+    int precision = PREC_FP64;
+    int half_mode = HALF_GHOST_FULL;
+    int ghost_mode = GHOST_FRESH;
+    int sync_calls = 1;
+    double dem_radius = 0.0;  // <= 0: contact cutoff / 2
+    int lj_shift = 0;
+
+    // host mirrors (filled by md_copy_data_from_accelerator)
+    std::vector<double> h_mass;
+    std::vector<double> h_pos, h_vel, h_force;  // AoS 3 doubles per atom (locals + ghosts)
+    bool host_valid = true;                     // host mirror is the newest copy
+    bool dev_valid = false;
+
+    // device state
+    DevBuf pos, vel, force, mass;      // double4 x3, double
+    DevBuf g_owner, g_src, g_code;     // ghost table
+    DevBuf cell_of, cell_slot, cell_count, cell_start, cell_parts;
+    DevBuf work_start, work;           // locals in cell order (work slots)
+    DevBuf nb_count, nb_list;
+    DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
+    DevBuf scan_blocks, scan_aux, counters, red, gbuf;
+
+    hipStream_t stream = nullptr;
+    KernelTimer timer;
+
+    // last thermo (device reductions)
+    double pe = 0.0, virial = 0.0;
+    long long pairs = 0;
+    // iteration counters reported by md_report_iterations
+    long long force_iters = 0, rsq_iters = 0, integration_iters = 0;
+};
+
+Sim& sim();
+bool sim_alive();
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// launchers (kernels live in the .hip translation units)
+// scan / compaction
+void exclusive_scan_int(const int* in, int* out, int n, int* total_dev, Sim& s);
+void scan_tiles(Sim& s, int* tiles, int ntiles, int* total_dev);  // one-block exclusive scan in place
+
+// borders / ghosts
+void build_ghosts(Sim& s);
+void refresh_ghosts(Sim& s);
+
+// binning
+void bin_particles(Sim& s);
+
+// neighbour lists
+void build_neighbor_lists(Sim& s, int half, double cutoff);
+
+// forces
+void compute_lj(Sim& s, int half, double rc, double eps, double sigma);
+void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu);
+void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs);
+
+void reduce_partials(Sim& s, const double* partials, int nblocks, int width, double* out);
+
+// integration
+void initial_integration(Sim& s, double dt);
+void final_integration(Sim& s, double dt);
+void enforce_pbc(Sim& s);
+double kinetic_sum(Sim& s);  // sum m v.v over locals (deterministic)
+
+}  // namespace tmd

@@ -1,0 +1,732 @@
+// tinyMD CPU oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// A from-scratch FP64 restatement of tinyMD's per-timestep path, used only by
+// tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the
+// checker.  The product library (molecular-dynamics_amd/csrc) never links,
+// loads or calls anything in this directory.
+//
+// Built with -O2 -fno-fast-math -ffp-contract=off so that every double
+// operation is the plain IEEE operation written here, in the order written.
+//
+// Each function cites the reference file:line it follows (paths relative to
+// the AnyDSL/molecular-dynamics tree).  Semantics follow the reference CPU
+// mapping: every loop runs in index order and an "atomic" add returns the old
+// value in index order (backend/mapping_cpu.impala:14-19), so cell lists,
+// ghost order and neighbour lists are the index-ordered ones.
+//
+// Pins (what checks each part; see tests/test_oracle_*.py):
+//   random / lattice / velocities  -> oracle/_ref build of initialize.h (bitwise)
+//                                     + Park-Miller 10,000th value
+//   LJ full-list force             -> oracle/_ref build of core/force_lj.cpp
+//                                     (bitwise) + brute force + lattice sums
+//   ghosts                         -> brute-force periodic image enumeration
+//   cells                          -> direct floor() check per atom
+//   neighbour lists                -> O(N^2) brute force with explicit images
+//   PE / pressure                  -> FCC lattice sums and the miniMD thermo
+//                                     line printed in results/single_gpu_v100.txt:39
+//   DEM                            -> closed-form restitution of a head-on pair
+//   velocity Verlet                -> free flight / harmonic closed forms,
+//                                     time reversibility
+//   adjoint                        -> central finite differences of the
+//                                     oracle's own loss (FP64)
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "tinymd_oracle.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Park-Miller "minimal standard" generator with Schrage's decomposition.
+// initialize.h:29-49
+double pm_random(int* idum) {
+    const int ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+    const double am = 1.0 / im;
+    int k = (*idum) / iq;
+    *idum = ia * (*idum - k * iq) - ir * k;
+    if (*idum < 0) *idum += im;
+    return am * (*idum);
+}
+
+struct Vec3 { double x, y, z; };
+
+enum { GHOST_FRESH = 0, GHOST_REF_LAG = 1 };
+enum { HALF_GHOST_FULL = 0, HALF_REF_LITERAL = 1 };
+
+struct Sim {
+    // geometry (core/runtime.impala:13-31, core/grid.impala:94-150)
+    double world[6];  // world AABB
+    double aabb[6];   // rank AABB extended by one cell spacing
+    double len[3];    // world lengths used for the periodic shift
+    double spacing;
+    int nx, ny, nz, ncells;
+
+    int n = 0;   // local atoms
+    int ng = 0;  // ghost atoms (slots n .. n+ng-1)
+
+    std::vector<double> m;      // masses      [n+ng]
+    std::vector<double> x;      // positions   [3*(n+ng)]
+    std::vector<double> v;      // velocities  [3*(n+ng)]
+    std::vector<double> f;      // forces      [3*n]
+
+    // ghost table: immediate source + per-direction flag (the reference's
+    // send_offsets / send_pbc), and the flattened owner + image.
+    std::vector<int> g_src;
+    std::vector<int> g_flag;    // 3 per ghost
+    std::vector<int> g_owner;
+    std::vector<int> g_img;     // 3 per ghost
+
+    // cell lists in CSR form, index-ordered inside each cell
+    std::vector<int> cell_start;     // [ncells+1]
+    std::vector<int> cell_particles; // [#binned]
+    std::vector<int> particle_cell;  // [n+ng], -1 if outside
+
+    // neighbour lists, row-major [n][cap] (backend/mapping_cpu.impala:1)
+    int nb_cap = 0;
+    std::vector<int> nb_count;
+    std::vector<int> nb_list;
+    int nb_half = 0;
+    int nb_half_mode = HALF_GHOST_FULL;
+
+    int ghost_mode = GHOST_FRESH;
+    int threads = 1;
+
+    // last force evaluation (full list: 1/2 per pair entry; half list:
+    // 1 per local-local pair, 1/2 per local-ghost pair)
+    double pe = 0.0, virial = 0.0;
+    long long pairs_in_cutoff = 0;
+};
+
+inline void set_threads(const Sim* s) {
+#ifdef _OPENMP
+    omp_set_num_threads(s->threads > 0 ? s->threads : 1);
+#else
+    (void)s;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Lattice generator: FCC sites at spacing a/2 with (i+j+k) even, walked in
+// 8^3 sub-boxes; per-site Park-Miller seeds.  initialize.h:91-201 (with the
+// is_within_aabb predicate of initialize.h:51-57 on the rank AABB).
+int lattice(const double aabb[6], const double rank_aabb[6], int half, double spacing,
+            double* mass, double* pos, double* vel, int cap) {
+    int nvertices[3], nxyz[3];
+    for (int d = 0; d < 3; ++d) {
+        nvertices[d] = (int)((aabb[d * 2 + 1] - aabb[d * 2 + 0]) / spacing);
+        nxyz[d] = nvertices[d] / 2;
+    }
+    int ilo = (int)(aabb[0] / spacing - 1), ihi = (int)(aabb[1] / spacing + 1);
+    int jlo = (int)(aabb[2] / spacing - 1), jhi = (int)(aabb[3] / spacing + 1);
+    int klo = (int)(aabb[4] / spacing - 1), khi = (int)(aabb[5] / spacing + 1);
+    ilo = std::max(ilo, 0); ihi = std::min(ihi, nvertices[0] - 1);
+    jlo = std::max(jlo, 0); jhi = std::min(jhi, nvertices[1] - 1);
+    klo = std::max(klo, 0); khi = std::min(khi, nvertices[2] - 1);
+
+    const int sub = 8;
+    int sx = 0, sy = 0, sz = 0, ox = 0, oy = 0, oz = 0;
+    int count = 0;
+    while (oz * sub <= khi) {
+        int k = oz * sub + sz, j = oy * sub + sy, i = ox * sub + sx;
+        if ((i + j + k) % 2 == 0 && i >= ilo && i <= ihi && j >= jlo && j <= jhi && k >= klo && k <= khi) {
+            double px = aabb[0] + i * spacing;
+            double py = aabb[2] + j * spacing;
+            double pz = aabb[4] + k * spacing;
+            bool inside = px >= rank_aabb[0] && px < rank_aabb[1] - 0.00001 &&
+                          py >= rank_aabb[2] && py < rank_aabb[3] - 0.00001 &&
+                          pz >= rank_aabb[4] && pz < rank_aabb[5] - 0.00001;
+            if (inside && (!half || py < aabb[2] + ((jhi - jlo) / 2) * spacing + 0.00001)) {
+                int seed = k * (2 * nxyz[1]) * (2 * nxyz[0]) + j * (2 * nxyz[0]) + i + 1;
+                double vx, vy, vz;
+                for (int r = 0; r < 5; ++r) pm_random(&seed);
+                vx = pm_random(&seed);
+                for (int r = 0; r < 5; ++r) pm_random(&seed);
+                vy = pm_random(&seed);
+                for (int r = 0; r < 5; ++r) pm_random(&seed);
+                vz = pm_random(&seed);
+                if (count < cap) {
+                    mass[count] = 1.0;
+                    pos[3 * count + 0] = px; pos[3 * count + 1] = py; pos[3 * count + 2] = pz;
+                    vel[3 * count + 0] = vx; vel[3 * count + 1] = vy; vel[3 * count + 2] = vz;
+                }
+                ++count;
+            }
+        }
+        sx++;
+        if (sx == sub) { sx = 0; sy++; }
+        if (sy == sub) { sy = 0; sz++; }
+        if (sz == sub) { sz = 0; ox++; }
+        if (ox * sub > ihi) { ox = 0; oy++; }
+        if (oy * sub > jhi) { oy = 0; oz++; }
+    }
+    return count;
+}
+
+// ---------------------------------------------------------------------------
+// Temperature, T = sum(v.v * m) / (3N - 3) with m = 1.  core/thermo.impala:1-16
+double temperature(const Sim* s) {
+    const double mass = 1.0;
+    double t_scale = 1.0 / (double)(s->n * 3 - 3);
+    double t = 0.0;
+    for (int i = 0; i < s->n; ++i) {
+        const double* vi = &s->v[3 * i];
+        t += (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]) * mass;
+    }
+    return t * t_scale;
+}
+
+// Periodic single-image wrap, strict comparisons.  comm/comm.impala:152-162
+inline void pbc_correct(const Sim* s, double* p) {
+    for (int d = 0; d < 3; ++d) {
+        double q = p[d];
+        if (p[d] < s->world[2 * d]) q += s->len[d];
+        if (p[d] > s->world[2 * d + 1]) q -= s->len[d];
+        p[d] = q;
+    }
+}
+
+// Cell of a position, flattened z-y-x.  core/grid.impala:406-426
+inline int cell_of(const Sim* s, const double* p) {
+    int i = (int)std::floor((p[0] - s->aabb[0]) / s->spacing);
+    int j = (int)std::floor((p[1] - s->aabb[2]) / s->spacing);
+    int k = (int)std::floor((p[2] - s->aabb[4]) / s->spacing);
+    return (k * s->ny + j) * s->nx + i;
+}
+
+// ---------------------------------------------------------------------------
+// borders(): 6 directions x+, x-, y+, y-, z+, z-; each scans locals plus the
+// ghosts of the previous dimensions, ghost slots are appended after each
+// +/- pair.  comm/comm.impala:105-128 (predicates), 528-642 (driver),
+// 723-734 (unpack), comm/lb.impala:88-100 (image flags), core/grid.impala:267-277.
+void borders(Sim* s) {
+    s->ng = 0;
+    s->g_src.clear(); s->g_flag.clear(); s->g_owner.clear(); s->g_img.clear();
+    const double dspacing = s->spacing * 2.0;
+    for (int d = 0; d < 3; ++d) {
+        const int scan_end = s->n + s->ng;
+        std::vector<int> src, flag;  // pending ghosts of this +/- pair
+        for (int side = 0; side < 2; ++side) {
+            for (int i = 0; i < scan_end; ++i) {
+                const double p = s->x[3 * i + d];
+                int fl;
+                if (side == 0) {  // "next" neighbour: particles near the upper face, image -1
+                    if (!(p > s->aabb[2 * d + 1] - dspacing)) continue;
+                    fl = (p > s->world[2 * d + 1] - s->spacing) ? -1 : 0;
+                } else {          // "prev" neighbour: particles near the lower face, image +1
+                    if (!(p < s->aabb[2 * d] + dspacing)) continue;
+                    fl = (p < s->world[2 * d] + s->spacing) ? 1 : 0;
+                }
+                src.push_back(i);
+                flag.push_back(fl);
+            }
+        }
+        const int add = (int)src.size();
+        const int base = s->n + s->ng;
+        s->m.resize(base + add);
+        s->x.resize(3 * (base + add));
+        s->v.resize(3 * (base + add));
+        for (int g = 0; g < add; ++g) {
+            const int i = src[g];
+            int fl[3] = {0, 0, 0};
+            fl[d] = flag[g];
+            const int slot = base + g;
+            s->m[slot] = s->m[i];
+            for (int e = 0; e < 3; ++e) s->x[3 * slot + e] = s->x[3 * i + e] + s->len[e] * (double)fl[e];
+            int owner, img[3];
+            if (i < s->n) {
+                owner = i; img[0] = fl[0]; img[1] = fl[1]; img[2] = fl[2];
+            } else {
+                const int gi = i - s->n;
+                owner = s->g_owner[gi];
+                for (int e = 0; e < 3; ++e) img[e] = s->g_img[3 * gi + e] + fl[e];
+            }
+            for (int e = 0; e < 3; ++e) s->v[3 * slot + e] = s->v[3 * owner + e];
+            s->g_src.push_back(i);
+            for (int e = 0; e < 3; ++e) s->g_flag.push_back(fl[e]);
+            s->g_owner.push_back(owner);
+            for (int e = 0; e < 3; ++e) s->g_img.push_back(img[e]);
+        }
+        s->ng += add;
+    }
+}
+
+// Per-step ghost refresh.  FRESH: pos[owner] + image*L (one add per
+// coordinate, equal bit for bit to what borders() writes).  REF_LAG: the
+// reference's two-phase pack/unpack, which reads the previous step's value of
+// ghost sources.  comm/comm.impala:334-388.
+void sync_ghosts(Sim* s) {
+    const int n = s->n;
+    if (s->ghost_mode == GHOST_REF_LAG) {
+        std::vector<double> buf(3 * (size_t)s->ng);
+        for (int g = 0; g < s->ng; ++g) {
+            const int i = s->g_src[g];
+            for (int e = 0; e < 3; ++e)
+                buf[3 * g + e] = s->x[3 * i + e] + s->len[e] * (double)s->g_flag[3 * g + e];
+        }
+        for (int g = 0; g < s->ng; ++g)
+            for (int e = 0; e < 3; ++e) s->x[3 * (n + g) + e] = buf[3 * g + e];
+    } else {
+        for (int g = 0; g < s->ng; ++g) {
+            const int o = s->g_owner[g];
+            for (int e = 0; e < 3; ++e)
+                s->x[3 * (n + g) + e] = s->x[3 * o + e] + s->len[e] * (double)s->g_img[3 * g + e];
+        }
+    }
+    for (int g = 0; g < s->ng; ++g) {
+        const int o = s->g_owner[g];
+        for (int e = 0; e < 3; ++e) s->v[3 * (n + g) + e] = s->v[3 * o + e];
+    }
+}
+
+// fill_cells(ghost=true): index-ordered cell lists over locals and ghosts;
+// atoms whose flattened cell index falls outside [0, ncells) are skipped.
+// core/grid.impala:280-321, 357.
+void distribute(Sim* s) {
+    const int nall = s->n + s->ng;
+    s->particle_cell.assign(nall, -1);
+    std::vector<int> count(s->ncells, 0);
+    for (int i = 0; i < nall; ++i) {
+        int c = cell_of(s, &s->x[3 * i]);
+        if (c >= 0 && c < s->ncells) { s->particle_cell[i] = c; count[c]++; }
+    }
+    s->cell_start.assign(s->ncells + 1, 0);
+    for (int c = 0; c < s->ncells; ++c) s->cell_start[c + 1] = s->cell_start[c] + count[c];
+    s->cell_particles.assign(s->cell_start[s->ncells], 0);
+    std::fill(count.begin(), count.end(), 0);
+    for (int i = 0; i < nall; ++i) {
+        int c = s->particle_cell[i];
+        if (c >= 0) s->cell_particles[s->cell_start[c] + count[c]++] = i;
+    }
+}
+
+// assemble_neighborlists: 27-cell stencil in z,y,x order with flat-index bound
+// checks only; skip self; half lists keep i > j (REF_LITERAL, the reference
+// filter) or i > j plus every ghost (GHOST_FULL); append when rsq <= r_nb^2;
+// grow the capacity to 2x the largest count and rebuild on overflow.
+// core/neighborlist.impala:1-78.
+void neighbors(Sim* s, int half, int half_mode, double cutoff) {
+    const int n = s->n, nx = s->nx, ny = s->ny, ncells = s->ncells;
+    const double cut2 = cutoff * cutoff;
+    s->nb_half = half; s->nb_half_mode = half_mode;
+    if (s->nb_cap <= 0) s->nb_cap = 1;
+    for (;;) {
+        s->nb_count.assign(n, 0);
+        s->nb_list.assign((size_t)n * s->nb_cap, 0);
+        int resize = 0;
+        const int cap = s->nb_cap;
+        set_threads(s);
+        #pragma omp parallel for schedule(dynamic, 256) reduction(max:resize)
+        for (int i = 0; i < n; ++i) {
+            const double* pi = &s->x[3 * i];
+            const int c = s->particle_cell[i];
+            int cnt = 0;
+            for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int nc = c + oz * nx * ny + oy * nx + ox;
+                if (nc < 0 || nc >= ncells) continue;
+                for (int k = s->cell_start[nc]; k < s->cell_start[nc + 1]; ++k) {
+                    const int j = s->cell_particles[k];
+                    if (nc == c && j == i) continue;
+                    if (half) {
+                        bool keep = (half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= n || i > j);
+                        if (!keep) continue;
+                    }
+                    const double* pj = &s->x[3 * j];
+                    double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+                    double rsq = dx * dx + dy * dy + dz * dz;
+                    if (rsq <= cut2) {
+                        if (cnt >= cap) { if (cnt > resize) resize = cnt; }
+                        else s->nb_list[(size_t)i * cap + cnt] = j;
+                        cnt++;
+                    }
+                }
+            }
+            s->nb_count[i] = cnt;
+        }
+        if (resize > 0) { s->nb_cap = resize * 2; continue; }
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pair kernels.  The LJ closure is core/runtime.impala:188-201 (force only);
+// energy U = 4 eps (sr12 - sr6) and the pair virial r.F = f * rsq are this
+// project's additions (the reference computes neither).  shift=1 selects the
+// shifted-force form F_sf(r) = F(r) - F(rc) (force and energy continuous at rc).
+struct LJ {
+    double rc2, rc, eps, sigma6, frc, urc;  // frc = |F(rc)|, urc = U(rc)
+    int shift;
+};
+
+inline LJ make_lj(double rc, double eps, double sigma, int shift) {
+    LJ p;
+    p.rc = rc; p.rc2 = rc * rc; p.eps = eps; p.shift = shift;
+    double s3 = sigma * (sigma * sigma);   // pow(sigma, 6) as utils/utilities.impala:27-36
+    p.sigma6 = s3 * s3;
+    double sr2 = 1.0 / p.rc2;
+    double sr6 = sr2 * sr2 * sr2 * p.sigma6;
+    p.frc = 48.0 * sr6 * (sr6 - 0.5) * sr2 * eps * rc;
+    p.urc = 4.0 * eps * sr6 * (sr6 - 1.0);
+    return p;
+}
+
+// returns f (force = f * del), pair energy u and pair virial w
+inline double lj_pair(const LJ& p, double rsq, double* u, double* w) {
+    double sr2 = 1.0 / rsq;
+    double sr6 = sr2 * sr2 * sr2 * p.sigma6;
+    double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * p.eps;
+    double e = 4.0 * p.eps * sr6 * (sr6 - 1.0);
+    if (p.shift) {
+        double r = std::sqrt(rsq);
+        f = f - p.frc / r;
+        e = e - p.urc + (r - p.rc) * p.frc;
+    }
+    *u = e;
+    *w = f * rsq;
+    return f;
+}
+
+// compute_potential driver: full lists accumulate in registers and store
+// once; half lists zero the forces and add +F to i and -F to local j.
+// core/compute.impala:79-126.
+void compute_lj(Sim* s, int half, const LJ& p) {
+    const int n = s->n;
+    s->f.assign(3 * (size_t)n, 0.0);
+    double pe = 0.0, vir = 0.0;
+    long long pairs = 0;
+    if (!half) {
+        const int cap = s->nb_cap;
+        set_threads(s);
+        std::vector<double> upi(n), wpi(n);
+        std::vector<int> npi(n);
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            const double* pi = &s->x[3 * i];
+            double fx = 0.0, fy = 0.0, fz = 0.0, ui = 0.0, wi = 0.0;
+            int np = 0;
+            for (int k = 0; k < s->nb_count[i]; ++k) {
+                const int j = s->nb_list[(size_t)i * cap + k];
+                const double* pj = &s->x[3 * j];
+                double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+                double rsq = dx * dx + dy * dy + dz * dz;
+                if (rsq < p.rc2) {
+                    double u, w;
+                    double f = lj_pair(p, rsq, &u, &w);
+                    fx = fx + dx * f; fy = fy + dy * f; fz = fz + dz * f;
+                    ui += u; wi += w; np++;
+                }
+            }
+            s->f[3 * i + 0] = fx; s->f[3 * i + 1] = fy; s->f[3 * i + 2] = fz;
+            upi[i] = ui; wpi[i] = wi; npi[i] = np;
+        }
+        for (int i = 0; i < n; ++i) { pe += upi[i]; vir += wpi[i]; pairs += npi[i]; }
+        s->pe = 0.5 * pe; s->virial = 0.5 * vir; s->pairs_in_cutoff = pairs;
+    } else {
+        const int cap = s->nb_cap;
+        for (int i = 0; i < n; ++i) {
+            const double* pi = &s->x[3 * i];
+            for (int k = 0; k < s->nb_count[i]; ++k) {
+                const int j = s->nb_list[(size_t)i * cap + k];
+                const double* pj = &s->x[3 * j];
+                double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+                double rsq = dx * dx + dy * dy + dz * dz;
+                if (rsq < p.rc2) {
+                    double u, w;
+                    double f = lj_pair(p, rsq, &u, &w);
+                    double Fx = dx * f, Fy = dy * f, Fz = dz * f;
+                    s->f[3 * i + 0] += Fx; s->f[3 * i + 1] += Fy; s->f[3 * i + 2] += Fz;
+                    if (j < n) {
+                        s->f[3 * j + 0] += -Fx; s->f[3 * j + 1] += -Fy; s->f[3 * j + 2] += -Fz;
+                        pe += u; vir += w;
+                    } else {
+                        pe += 0.5 * u; vir += 0.5 * w;
+                    }
+                    pairs++;
+                }
+            }
+        }
+        s->pe = pe; s->virial = vir; s->pairs_in_cutoff = pairs;
+    }
+}
+
+// Spring-dashpot DEM contact.  Follows the structure of core/runtime.impala:203-247
+// with the contact condition, sign conventions and radius corrected (the
+// reference's closure is identically zero): overlap delta = 2R - r > 0,
+// n = del/r, F_n = (k delta - g_n v_n) n, F_t = -min(g_t |v_t|, mu |F_n|) t.
+struct DEM { double rc2, radius, gn, gt, k, mu; };
+
+inline void dem_pair(const DEM& p, const double* del, double rsq, const double* vi, const double* vj, double* F) {
+    F[0] = F[1] = F[2] = 0.0;
+    double d2 = 2.0 * p.radius;
+    if (rsq < d2 * d2) {
+        double r = std::sqrt(rsq);
+        double delta = d2 - r;
+        double nrm[3] = {del[0] / r, del[1] / r, del[2] / r};
+        double vr[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+        double vn = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
+        double vt[3] = {vr[0] - vn * nrm[0], vr[1] - vn * nrm[1], vr[2] - vn * nrm[2]};
+        double vtl = std::sqrt(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+        double fn = p.k * delta - p.gn * vn;
+        double ft = std::min(p.gt * vtl, p.mu * std::fabs(fn));
+        double inv = vtl > 0.0 ? ft / vtl : 0.0;
+        for (int e = 0; e < 3; ++e) F[e] = fn * nrm[e] - inv * vt[e];
+    }
+}
+
+void compute_dem(Sim* s, int half, const DEM& p) {
+    const int n = s->n, cap = s->nb_cap;
+    s->f.assign(3 * (size_t)n, 0.0);
+    long long pairs = 0;
+    for (int i = 0; i < n; ++i) {
+        const double* pi = &s->x[3 * i];
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < s->nb_count[i]; ++k) {
+            const int j = s->nb_list[(size_t)i * cap + k];
+            const double* pj = &s->x[3 * j];
+            double del[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+            double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+            if (rsq < p.rc2) {
+                double F[3];
+                dem_pair(p, del, rsq, &s->v[3 * i], &s->v[3 * j], F);
+                if (!half) {
+                    for (int e = 0; e < 3; ++e) acc[e] = acc[e] + F[e];
+                } else {
+                    for (int e = 0; e < 3; ++e) s->f[3 * i + e] += F[e];
+                    if (j < n) for (int e = 0; e < 3; ++e) s->f[3 * j + e] += -F[e];
+                }
+                pairs++;
+            }
+        }
+        if (!half) for (int e = 0; e < 3; ++e) s->f[3 * i + e] = acc[e];
+    }
+    s->pe = 0.0; s->virial = 0.0; s->pairs_in_cutoff = pairs;
+}
+
+// Velocity Verlet, unit mass.  core/compute.impala:175-194
+void initial_integration(Sim* s, double dt) {
+    for (int i = 0; i < s->n; ++i)
+        for (int e = 0; e < 3; ++e) {
+            s->v[3 * i + e] = s->v[3 * i + e] + s->f[3 * i + e] * (0.5 * dt);
+            s->x[3 * i + e] = s->x[3 * i + e] + s->v[3 * i + e] * dt;
+        }
+}
+
+void final_integration(Sim* s, double dt) {
+    for (int i = 0; i < s->n; ++i)
+        for (int e = 0; e < 3; ++e) s->v[3 * i + e] = s->v[3 * i + e] + s->f[3 * i + e] * (0.5 * dt);
+}
+
+void enforce_pbc(Sim* s) {
+    for (int i = 0; i < s->n; ++i) pbc_correct(s, &s->x[3 * i]);
+}
+
+void rebuild(Sim* s, int half, int half_mode, double cutoff) {
+    // exchange_particles (single rank): nghost = 0.  comm/comm.impala:392-396
+    s->ng = 0;
+    s->m.resize(s->n); s->x.resize(3 * s->n); s->v.resize(3 * s->n);
+    enforce_pbc(s);
+    borders(s);
+    distribute(s);
+    neighbors(s, half, half_mode, cutoff);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API (ctypes)
+extern "C" {
+
+double orc_pm_random(int* seed) { return pm_random(seed); }
+
+int orc_lattice(const double* aabb, const double* rank_aabb, int half, double spacing,
+                double* mass, double* pos, double* vel, int cap) {
+    return lattice(aabb, rank_aabb, half, spacing, mass, pos, vel, cap);
+}
+
+// md_initialize_grid: extend the rank AABB by one cell spacing and size the
+// cell grid.  core/runtime.impala:7-37, core/grid.impala:94-150, 215-249.
+void* orc_create(const double* m, const double* pos, const double* vel, int n,
+                 const double* aabb, const double* rank_aabb, double cell_spacing, int nb_cap) {
+    Sim* s = new Sim();
+    for (int d = 0; d < 6; ++d) s->world[d] = aabb[d];
+    for (int d = 0; d < 3; ++d) {
+        s->aabb[2 * d] = rank_aabb[2 * d] - cell_spacing;
+        s->aabb[2 * d + 1] = rank_aabb[2 * d + 1] + cell_spacing;
+        s->len[d] = aabb[2 * d + 1] - aabb[2 * d];
+    }
+    s->spacing = cell_spacing;
+    s->nx = (int)std::floor((s->aabb[1] - s->aabb[0] + (cell_spacing / 2.0)) / cell_spacing) + 1;
+    s->ny = (int)std::floor((s->aabb[3] - s->aabb[2] + (cell_spacing / 2.0)) / cell_spacing) + 1;
+    s->nz = (int)std::floor((s->aabb[5] - s->aabb[4] + (cell_spacing / 2.0)) / cell_spacing) + 1;
+    s->ncells = s->nx * s->ny * s->nz;
+    s->n = n;
+    s->m.assign(m, m + n);
+    s->x.assign(pos, pos + 3 * (size_t)n);
+    s->v.assign(vel, vel + 3 * (size_t)n);
+    s->f.assign(3 * (size_t)n, 0.0);
+    s->nb_cap = nb_cap > 0 ? nb_cap : 1;
+    return s;
+}
+
+void orc_destroy(void* h) { delete (Sim*)h; }
+void orc_set_threads(void* h, int t) { ((Sim*)h)->threads = t; }
+void orc_set_ghost_mode(void* h, int mode) { ((Sim*)h)->ghost_mode = mode; }
+
+void orc_grid_info(void* h, int* out /* nx ny nz ncells n ng nb_cap */) {
+    Sim* s = (Sim*)h;
+    out[0] = s->nx; out[1] = s->ny; out[2] = s->nz; out[3] = s->ncells;
+    out[4] = s->n; out[5] = s->ng; out[6] = s->nb_cap;
+}
+
+// md_create_velocity: remove the mean, rescale to T_request.
+// core/runtime.impala:158-186
+void orc_create_velocity(void* h, double t_request) {
+    Sim* s = (Sim*)h;
+    double vt[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < s->n; ++i) for (int e = 0; e < 3; ++e) vt[e] = vt[e] + s->v[3 * i + e];
+    for (int e = 0; e < 3; ++e) vt[e] = vt[e] / (double)s->n;
+    for (int i = 0; i < s->n; ++i) for (int e = 0; e < 3; ++e) s->v[3 * i + e] = s->v[3 * i + e] - vt[e];
+    double t = temperature(s);
+    double factor = std::sqrt(t_request / t);
+    for (int i = 0; i < s->n; ++i) for (int e = 0; e < 3; ++e) s->v[3 * i + e] = s->v[3 * i + e] * factor;
+}
+
+void orc_exchange(void* h) {
+    Sim* s = (Sim*)h;
+    s->ng = 0;
+    s->m.resize(s->n); s->x.resize(3 * s->n); s->v.resize(3 * s->n);
+}
+void orc_enforce_pbc(void* h) { enforce_pbc((Sim*)h); }
+void orc_borders(void* h) { borders((Sim*)h); }
+void orc_distribute(void* h) { distribute((Sim*)h); }
+void orc_neighbors(void* h, int half, int half_mode, double cutoff) { neighbors((Sim*)h, half, half_mode, cutoff); }
+void orc_sync_ghosts(void* h) { sync_ghosts((Sim*)h); }
+void orc_initial_integration(void* h, double dt) { initial_integration((Sim*)h, dt); }
+void orc_final_integration(void* h, double dt) { final_integration((Sim*)h, dt); }
+
+void orc_lj(void* h, int half, double rc, double eps, double sigma, int shift) {
+    compute_lj((Sim*)h, half, make_lj(rc, eps, sigma, shift));
+}
+
+void orc_dem(void* h, int half, double rc, double radius, double gn, double gt, double k, double mu) {
+    DEM p{rc * rc, radius, gn, gt, k, mu};
+    compute_dem((Sim*)h, half, p);
+}
+
+// thermo: out = {T, KE, PE, virial, P, pairs_in_cutoff}; P = (sum m v^2 + W)/(3V)
+void orc_thermo(void* h, double* out) {
+    Sim* s = (Sim*)h;
+    double t = temperature(s);
+    double sum_mv2 = t * (double)(3 * s->n - 3);
+    double vol = s->len[0] * s->len[1] * s->len[2];
+    out[0] = t;
+    out[1] = 0.5 * sum_mv2;
+    out[2] = s->pe;
+    out[3] = s->virial;
+    out[4] = (sum_mv2 + s->virial) / (3.0 * vol);
+    out[5] = (double)s->pairs_in_cutoff;
+}
+
+void orc_get_state(void* h, double* pos, double* vel, double* force) {
+    Sim* s = (Sim*)h;
+    if (pos) std::memcpy(pos, s->x.data(), sizeof(double) * 3 * s->n);
+    if (vel) std::memcpy(vel, s->v.data(), sizeof(double) * 3 * s->n);
+    if (force) std::memcpy(force, s->f.data(), sizeof(double) * 3 * s->n);
+}
+
+void orc_set_state(void* h, const double* pos, const double* vel) {
+    Sim* s = (Sim*)h;
+    if (pos) std::memcpy(s->x.data(), pos, sizeof(double) * 3 * s->n);
+    if (vel) std::memcpy(s->v.data(), vel, sizeof(double) * 3 * s->n);
+}
+
+void orc_get_ghosts(void* h, int* owner, int* img, int* src, int* flag, double* pos) {
+    Sim* s = (Sim*)h;
+    for (int g = 0; g < s->ng; ++g) {
+        if (owner) owner[g] = s->g_owner[g];
+        if (src) src[g] = s->g_src[g];
+        for (int e = 0; e < 3; ++e) {
+            if (img) img[3 * g + e] = s->g_img[3 * g + e];
+            if (flag) flag[3 * g + e] = s->g_flag[3 * g + e];
+            if (pos) pos[3 * g + e] = s->x[3 * (s->n + g) + e];
+        }
+    }
+}
+
+// cell CSR: start[ncells+1], particles[start[ncells]], particle_cell[n+ng]
+void orc_get_cells(void* h, int* start, int* particles, int* pcell) {
+    Sim* s = (Sim*)h;
+    if (start) std::memcpy(start, s->cell_start.data(), sizeof(int) * (s->ncells + 1));
+    if (particles) std::memcpy(particles, s->cell_particles.data(), sizeof(int) * s->cell_particles.size());
+    if (pcell) std::memcpy(pcell, s->particle_cell.data(), sizeof(int) * s->particle_cell.size());
+}
+
+// neighbour lists as counts[n] and row-major list[n*stride] (stride >= nb_cap)
+void orc_get_neighbors(void* h, int* counts, int* list, int stride) {
+    Sim* s = (Sim*)h;
+    for (int i = 0; i < s->n; ++i) {
+        counts[i] = s->nb_count[i];
+        if (list)
+            for (int k = 0; k < s->nb_count[i] && k < stride; ++k) list[(size_t)i * stride + k] = s->nb_list[(size_t)i * s->nb_cap + k];
+    }
+}
+
+// The main.cpp timestep loop for LJ (main.cpp:715-776), for timing and
+// trajectory checks.  Returns the number of rebuilds.
+int orc_run_lj(void* h, int steps, int reneigh_every, double dt, int half, int half_mode,
+               double rc, double skin, double eps, double sigma, int shift) {
+    Sim* s = (Sim*)h;
+    LJ p = make_lj(rc, eps, sigma, shift);
+    int rebuilds = 0;
+    for (int j = 0; j < steps; ++j) {
+        initial_integration(s, dt);
+        if ((j + 1) % reneigh_every == 0) {
+            rebuild(s, half, half_mode, rc + skin);
+            rebuilds++;
+        } else {
+            sync_ghosts(s);
+        }
+        compute_lj(s, half, p);
+        final_integration(s, dt);
+    }
+    return rebuilds;
+}
+
+int orc_run_dem(void* h, int steps, int reneigh_every, double dt, int half, int half_mode,
+                double rc, double skin, double radius, double gn, double gt, double k, double mu) {
+    Sim* s = (Sim*)h;
+    DEM p{rc * rc, radius, gn, gt, k, mu};
+    int rebuilds = 0;
+    for (int j = 0; j < steps; ++j) {
+        initial_integration(s, dt);
+        if ((j + 1) % reneigh_every == 0) {
+            rebuild(s, half, half_mode, rc + skin);
+            rebuilds++;
+        } else {
+            sync_ghosts(s);
+        }
+        compute_dem(s, half, p);
+        final_integration(s, dt);
+    }
+    return rebuilds;
+}
+
+// Initial setup as main.cpp:691-700 (exchange, borders, distribute, lists).
+void orc_setup(void* h, int half, int half_mode, double cutoff) {
+    Sim* s = (Sim*)h;
+    s->ng = 0;
+    borders(s);
+    distribute(s);
+    neighbors(s, half, half_mode, cutoff);
+}
+
+}  // extern "C"

@@ -1,0 +1,257 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer/index work (ghost tables, cell lists,
+neighbour lists) must be bit-exact; FP64 full-list forces and trajectories
+are bit-exact too (same operation order, no contraction); half lists, the
+FP32-mixed path and reductions have stated tolerances."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from common import RC, SKIN, announce, force_error, make_pair, perturbed_fcc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tm():
+    import tinymd_amd
+    name, cus = tinymd_amd.device_info()
+    if cus == 0:
+        pytest.fail("no HIP device visible to libtinymd_amd")
+    announce(f"device {name} with {cus} CUs")
+    return tinymd_amd
+
+
+def setup_both(sim, orc, half=False, half_mode="ghost_full"):
+    hm = po.HALF_GHOST_FULL if half_mode == "ghost_full" else po.HALF_REF_LITERAL
+    sim.setup(half, RC, SKIN)
+    orc.setup(half, hm, RC + SKIN)
+
+
+def test_lattice_matches_oracle(tm):
+    announce("lattice 8x6x5 + create_velocity")
+    m, x, v = tm.generate_fcc(8, 6, 5)
+    aabb, m2, x2, v2 = po.lattice(8, 6, 5)
+    assert np.array_equal(x, x2) and np.array_equal(v, v2) and np.array_equal(m, m2)
+    sim = tm.Simulation(m, x, v, aabb, RC + SKIN)
+    sim.create_velocity(1.44)
+    sim.copy_to_accelerator()
+    orc = po.OracleSim(m2, x2, v2, aabb, RC + SKIN)
+    orc.create_velocity(1.44)
+    _, _, vg, _ = sim.state()
+    _, vo, _ = orc.state()
+    assert np.array_equal(vg, vo)
+
+
+@pytest.mark.parametrize("shape,amp", [((8, 8, 8), 0.0), ((6, 7, 9), 0.15), ((5, 5, 5), 0.3)])
+def test_ghosts_cells_exact(tm, shape, amp):
+    announce(f"ghosts+cells {shape} amp={amp}")
+    aabb, m, x, v = perturbed_fcc(*shape, amp=amp, seed=1)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc)
+    assert sim.nghost == orc.ng
+    gt, go = sim.ghost_table(), orc.ghosts()
+    for key in ("owner", "img", "src", "flag"):
+        assert np.array_equal(gt[key], go[key]), key
+    _, gx, _, _ = sim.ghost_state()
+    assert np.array_equal(gx, go["pos"])
+    st, parts, pcell = sim.cells()
+    so, po_, pco = orc.cells()
+    assert np.array_equal(st, so) and np.array_equal(parts, po_) and np.array_equal(pcell, pco)
+
+
+@pytest.mark.parametrize("half,half_mode", [(False, "ghost_full"), (True, "ghost_full"), (True, "ref_literal")])
+def test_neighbor_lists_exact(tm, half, half_mode):
+    announce(f"neighbour lists half={half} mode={half_mode}")
+    aabb, m, x, v = perturbed_fcc(7, 8, 6, amp=0.2, seed=2)
+    sim, orc = make_pair(aabb, m, x, v, half_mode=half_mode)
+    setup_both(sim, orc, half, half_mode)
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert np.array_equal(cg, co)
+    n = len(cg)
+    for i in range(n):
+        assert np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]), i
+    if half and half_mode == "ref_literal":
+        # the reference's literal filter keeps no ghost (index >= n)
+        assert all(np.all(lo[i, :co[i]] < n) for i in range(n))
+
+
+def test_list_capacity_retry(tm):
+    announce("list capacity 1 -> grow and retry")
+    aabb, m, x, v = perturbed_fcc(6, amp=0.1, seed=3)
+    sim, orc = make_pair(aabb, m, x, v, nb_cap=1)
+    setup_both(sim, orc)
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert np.array_equal(cg, co)
+    for i in range(len(cg)):
+        assert np.array_equal(lg[i, :cg[i]], lo[i, :co[i]])
+
+
+def test_lj_full_fp64_bitwise(tm):
+    announce("LJ full FP64 forces, bit-exact")
+    aabb, m, x, v = perturbed_fcc(8, amp=0.1, seed=4)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    assert np.array_equal(fg, fo)
+
+
+@pytest.mark.parametrize("half_mode", ["ghost_full", "ref_literal"])
+def test_lj_half_fp64(tm, half_mode):
+    announce(f"LJ half FP64 forces mode={half_mode}")
+    aabb, m, x, v = perturbed_fcc(8, amp=0.1, seed=5)
+    sim, orc = make_pair(aabb, m, x, v, half_mode=half_mode)
+    hm = po.HALF_GHOST_FULL if half_mode == "ghost_full" else po.HALF_REF_LITERAL
+    setup_both(sim, orc, True, half_mode)
+    sim.lennard_jones(True, RC)
+    orc.lj(True, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    rms, mx = force_error(fg, fo)
+    assert rms < 1e-12 and mx < 1e-11, (rms, mx)
+    if half_mode == "ghost_full":
+        # each periodic pair once: equals the full-list force
+        orc2 = po.OracleSim(m, x, v, aabb, RC + SKIN)
+        orc2.setup(False, hm, RC + SKIN)
+        orc2.lj(False, RC)
+        _, _, ff = orc2.state()
+        rms, mx = force_error(fg, ff)
+        assert rms < 1e-12 and mx < 1e-11, (rms, mx)
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_lj_fp32_mixed_error(tm, half):
+    announce(f"LJ FP32-mixed error half={half}")
+    aabb, m, x, v = perturbed_fcc(10, amp=0.15, seed=6)
+    sim, orc = make_pair(aabb, m, x, v, precision="fp32mixed")
+    setup_both(sim, orc, half)
+    sim.lennard_jones(half, RC)
+    orc.lj(half, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    rms, mx = force_error(fg, fo)
+    # <= 1e-5 relative force error vs FP64
+    assert rms < 1e-5 and mx < 1e-5 * 10, (rms, mx)
+    announce(f"  fp32-mixed rms={rms:.3e} max={mx:.3e}")
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_thermo_matches_oracle(tm, half):
+    announce(f"thermo half={half}")
+    aabb, m, x, v = perturbed_fcc(8, amp=0.1, seed=7)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc, half)
+    sim.lennard_jones(half, RC)
+    orc.lj(half, RC)
+    tg, to = sim.thermo(RC), orc.thermo()
+    for key in ("T", "KE", "PE", "virial", "P"):
+        assert abs(tg[key] - to[key]) <= 1e-12 * max(1.0, abs(to[key])), (key, tg[key], to[key])
+    assert tg["pairs"] == to["pairs"]
+
+
+def test_trajectory_bitwise_abi_and_runner(tm):
+    announce("trajectory 6^3, 60 steps, ABI loop and in-library loop vs oracle")
+    aabb, m, x, v = perturbed_fcc(6, amp=0.0, seed=0)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    sim.run_abi_lj(30, reneigh=20, dt=0.005)
+    orc.run_lj(30, reneigh=20, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    sim.run_lj(30, reneigh=20, dt=0.005)
+    orc.run_lj(30, reneigh=20, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+
+
+def test_ref_lag_ghosts_trajectory(tm):
+    announce("REF_LAG ghost refresh trajectory vs oracle")
+    aabb, m, x, v = perturbed_fcc(6, amp=0.05, seed=8)
+    sim, orc = make_pair(aabb, m, x, v, ghost_mode="ref_lag")
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    sim.run_lj(25, reneigh=20, dt=0.005)
+    orc.run_lj(25, reneigh=20, dt=0.005)
+    _, xg, vg, _ = sim.state()
+    xo, vo, _ = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo)
+
+
+def test_half_list_trajectory_close(tm):
+    announce("half-list trajectory vs oracle (tolerance)")
+    aabb, m, x, v = perturbed_fcc(6, amp=0.05, seed=9)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc, True)
+    sim.lennard_jones(True, RC)
+    orc.lj(True, RC)
+    sim.run_lj(40, reneigh=20, dt=0.005, half=True)
+    orc.run_lj(40, reneigh=20, dt=0.005, half=True)
+    _, xg, vg, _ = sim.state()
+    xo, vo, _ = orc.state()
+    assert np.max(np.abs(xg - xo)) < 1e-9 and np.max(np.abs(vg - vo)) < 1e-8
+
+
+def test_determinism_half_and_full(tm):
+    announce("bitwise run-to-run determinism")
+    aabb, m, x, v = perturbed_fcc(7, amp=0.1, seed=10)
+    outs = []
+    for _ in range(2):
+        sim, _ = make_pair(aabb, m, x, v)
+        sim.setup(True, RC, SKIN)
+        sim.lennard_jones(True, RC)
+        sim.run_lj(20, reneigh=10, half=True)
+        outs.append(sim.state()[1:])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("half", [True, False])
+def test_dem_matches_oracle(tm, half):
+    announce(f"DEM granular gas 8^3 half={half}")
+    m, x, v = tm.generate_granular_gas(8, seed=7, vmax=1.0)
+    aabb = np.array([0.0, 8 * 1.6796, 0.0, 8 * 1.6796, 0.0, 8 * 1.6796])
+    # compress the lattice so spheres of diameter 1.0 overlap
+    x = x * 0.55
+    aabb = aabb * 0.55
+    sim, orc = make_pair(aabb, m, x, v, cutoff=1.3)
+    sim.setup(half, 1.0, 0.3)
+    orc.setup(half, po.HALF_GHOST_FULL, 1.3)
+    sim.dem(half, 1.0, 0.5, 0.2, 100.0, 0.3)
+    orc.dem(half, 1.0, 0.5, 0.5, 0.2, 100.0, 0.3)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    assert np.abs(fo).max() > 1.0  # contacts exist
+    rms, mx = force_error(fg, fo)
+    assert rms < 1e-12 and mx < 1e-11, (rms, mx)
+    sim.run_dem(30, reneigh=10, dt=0.001, half=half)
+    orc.run_dem(30, reneigh=10, dt=0.001, half=half)
+    _, xg, vg, _ = sim.state()
+    xo, vo, _ = orc.state()
+    assert np.max(np.abs(xg - xo)) < 1e-10 and np.max(np.abs(vg - vo)) < 1e-9
+
+
+def test_full_size_lattice_invariants(tm):
+    """80^3 (2,048,000 atoms): size-independent invariants at t = 0."""
+    announce("80^3 lattice invariants")
+    sim = tm.Simulation.fcc(80)
+    sim.setup(False, RC, SKIN)
+    assert sim.nghost == 280731
+    counts, _ = sim.neighbor_lists(with_lists=False)
+    assert np.all(counts == 78)
+    sim.lennard_jones(False, RC)
+    th = sim.thermo(RC)
+    _, _, _, f = sim.state()
+    assert np.abs(f).max() < 1e-10  # perfect lattice: F = 0 by symmetry
+    assert abs(th["PE"] / sim.nlocal - (-6.7733178)) < 5e-7
+    assert abs(th["T"] - 1.44) < 1e-12
+    sim.close()
