@@ -64,7 +64,7 @@ def run_variant(tm, precision, half, steps, warmup, profile=False):
         ms, cnt = sim.profile_get("force")
         out["force_ms_avg"] = ms / max(cnt, 1)
         out["force_launches"] = cnt
-        for k in ("integrate1", "integrate2", "ghost_sync"):
+        for k in ("integrate1", "integrate2", "ghost_sync", "mirror", "neighbor", "bin", "borders"):
             kms, kc = sim.profile_get(k)
             out[k + "_ms_avg"] = kms / max(kc, 1)
         sim.profile(False)

@@ -40,6 +40,9 @@ void tmd_set_dem_radius(double radius);
 /* 1 = shifted-force LJ, F(r) - F(rc), with energy shifted to match
  * (continuous force at rc; used for the differentiable runs). */
 void tmd_set_lj_shift(int on);
+/* Cells per tile of the list/force kernels (one block per tile; 0 0 0 =
+ * automatic, about 480 atoms per tile).  Takes effect at the next binning. */
+void tmd_set_tile(int tx, int ty, int tz);
 
 /* ---- thermo (deterministic device reductions) ------------------------------
  * out = {T, KE, PE, W, P, pairs}: T = sum v^2 / (3N-3) (core/thermo.impala:1-16),
@@ -70,7 +73,8 @@ int tmd_run_dem(int steps, int reneigh_every, double dt, bool half, double cutof
 
 /* ---- profiling: HIP events around each launch on the library stream -------- */
 /* kernel ids: 0 force, 1 initial integration, 2 final integration,
- * 3 ghost refresh, 4 neighbour list, 5 binning, 6 borders, 7 thermo, 8 adjoint */
+ * 3 ghost refresh, 4 neighbour list, 5 binning, 6 borders, 7 thermo, 8 adjoint,
+ * 9 cell-ordered position mirror */
 void tmd_profile_enable(int on);
 void tmd_profile_get(int kernel, double* total_ms, long long* count);
 void tmd_profile_reset(void);

@@ -5,6 +5,7 @@
 // mapping (core/grid.impala:280-321; cell math core/grid.impala:406-426).
 #include "tmd_internal.hpp"
 #include "tmd_device.hpp"
+#include "tmd_window.hpp"
 
 namespace tmd {
 
@@ -62,15 +63,69 @@ __global__ void k_bin_sort(const int* __restrict__ start, int ncells, int n, int
     local_count[c] = nl;
 }
 
-// Work slots: the locals in cell order.  Neighbour-list rows and the force
-// kernels walk atoms in this order (coalesced list reads, cell-local
-// gathers); storage keeps the reference's atom index order.
-__global__ void k_work_fill(const int* __restrict__ start, const int* __restrict__ work_start, int ncells,
-                            const int* __restrict__ parts, int* __restrict__ work) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const int b = start[c], wb = work_start[c], nl = work_start[c + 1] - wb;
-    for (int a = 0; a < nl; ++a) work[wb + a] = parts[b + a];
+// Tile work order (see tmd_window.hpp): one thread per tile counts the
+// tile's locals and the size of its window.
+__global__ void k_tile_stats(const int* __restrict__ start, const int* __restrict__ local_count, TileGeom g, int ntiles,
+                             int* __restrict__ tile_count, int* __restrict__ max_window) {
+    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= ntiles) return;
+    const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
+    const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+    const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
+    int cnt = 0;
+    for (int z = cz0; z < cz1; ++z)
+        for (int y = cy0; y < cy1; ++y)
+            for (int x = cx0; x < cx1; ++x) cnt += local_count[(z * g.ny + y) * g.nx + x];
+    tile_count[tile] = cnt;
+    int S = 0;
+    const int xlo = cx0 - 1, xhi = min(cx0 + g.tx, g.nx - 1);
+    for (int z = cz0 - 1; z <= cz0 + g.tz; ++z)
+        for (int y = cy0 - 1; y <= cy0 + g.ty; ++y)
+            if (y < g.ny && z < g.nz) {
+                const int row = (z * g.ny + y) * g.nx;
+                S += start[row + xhi + 1] - start[row + xlo];
+            }
+    if (cnt > 0) atomicMax(max_window, S);
+}
+
+// One thread per tile: the tile's locals in (z, y, x) cell order.
+__global__ void k_tile_fill(const int* __restrict__ start, const int* __restrict__ local_count, const int* __restrict__ tile_start,
+                            TileGeom g, int ntiles, const int* __restrict__ parts, int* __restrict__ work) {
+    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= ntiles) return;
+    const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
+    const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+    const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
+    int w = tile_start[tile];
+    for (int z = cz0; z < cz1; ++z)
+        for (int y = cy0; y < cy1; ++y)
+            for (int x = cx0; x < cx1; ++x) {
+                const int c = (z * g.ny + y) * g.nx + x;
+                const int b = start[c], nl = local_count[c];
+                for (int a = 0; a < nl; ++a) work[w++] = parts[b + a];
+            }
+}
+
+// Cells per tile: about 480 locals per tile (one 512-thread block), with
+// the window (tile + 1 cell on each side) inside the LDS budget.  An
+// explicit choice (tmd_set_tile) overrides.
+static void choose_tiles(Sim& s) {
+    const Geom& G = s.g;
+    int t[3] = {s.tile_req[0], s.tile_req[1], s.tile_req[2]};
+    if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) {
+        const double inner = (double)std::max(G.nx - 2, 1) * std::max(G.ny - 2, 1) * std::max(G.nz - 2, 1);
+        const double rho = std::max((double)s.n / inner, 1e-3);
+        int T = (int)std::lround(std::cbrt(480.0 / rho));
+        T = std::max(1, std::min(T, kMaxTile));
+        while (T > 1 && (T + 2.0) * (T + 2.0) * (T + 2.0) * rho * 28.0 > 0.85 * kLdsBudget) --T;
+        t[0] = t[1] = t[2] = T;
+    }
+    TileGeom& g = s.tiles;
+    g.nx = G.nx; g.ny = G.ny; g.nz = G.nz;
+    g.tx = std::min(t[0], kMaxTile); g.ty = std::min(t[1], kMaxTile); g.tz = std::min(t[2], kMaxTile);
+    g.ntx = std::max(1, div_up(G.nx - 2, g.tx));
+    g.nty = std::max(1, div_up(G.ny - 2, g.ty));
+    g.ntz = std::max(1, div_up(G.nz - 2, g.tz));
 }
 
 void bin_particles(Sim& s) {
@@ -79,7 +134,6 @@ void bin_particles(Sim& s) {
     const Geom& G = s.g;
     s.cell_count.reserve(sizeof(int) * (size_t)G.ncells, false, s.stream);
     s.cell_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
-    s.work_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
     s.work.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
     if (has_error()) return;
     BinArgs a{G.aabb[0], G.aabb[2], G.aabb[4], G.spacing, G.nx, G.ny, G.ncells, nall};
@@ -94,10 +148,24 @@ void bin_particles(Sim& s) {
                            (const int*)s.cell_slot.as<int>(), (const int*)s.cell_start.as<int>(), nall, s.cell_parts.as<int>());
     hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(), G.ncells,
                        s.n, s.cell_parts.as<int>(), s.cell_count.as<int>());
-    exclusive_scan_int(s.cell_count.as<int>(), s.work_start.as<int>(), G.ncells, s.work_start.as<int>() + G.ncells, s);
-    hipLaunchKernelGGL(k_work_fill, dim3(div_up(G.ncells, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(),
-                       (const int*)s.work_start.as<int>(), G.ncells, (const int*)s.cell_parts.as<int>(), s.work.as<int>());
+    // tile work order
+    choose_tiles(s);
+    const TileGeom tg = s.tiles;
+    const int ntiles = tg.ntx * tg.nty * tg.ntz;
+    s.tile_start.reserve(sizeof(int) * ((size_t)ntiles + 1), false, s.stream);
+    s.work_start.reserve(sizeof(int) * ((size_t)ntiles + 1), false, s.stream);  // per-tile counts
+    if (has_error()) return;
+    int* counters = s.counters.as<int>();
+    TMD_HIP(hipMemsetAsync(counters + 6, 0, sizeof(int), s.stream));
+    hipLaunchKernelGGL(k_tile_stats, dim3(div_up(ntiles, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(),
+                       (const int*)s.cell_count.as<int>(), tg, ntiles, s.work_start.as<int>(), counters + 6);
+    exclusive_scan_int(s.work_start.as<int>(), s.tile_start.as<int>(), ntiles, s.tile_start.as<int>() + ntiles, s);
+    hipLaunchKernelGGL(k_tile_fill, dim3(div_up(ntiles, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(),
+                       (const int*)s.cell_count.as<int>(), (const int*)s.tile_start.as<int>(), tg, ntiles,
+                       (const int*)s.cell_parts.as<int>(), s.work.as<int>());
     TMD_HIP(hipGetLastError());
+    s.windows_valid = 0;
+    build_windows(s);
 }
 
 }  // namespace tmd

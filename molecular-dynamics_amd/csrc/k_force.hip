@@ -1,7 +1,12 @@
 // Pair-force kernels: Lennard-Jones and spring-dashpot DEM, full and half
-// lists, FP64 and FP32-mixed.  One work slot (local atom, cell order) per
-// lane; the neighbour indices of a wavefront are read as 256-byte rows of
-// the tiled list (see k_nlist.hip).
+// lists, FP64 and FP32-mixed.
+//
+// One block per tile of cells, one lane per local atom of the tile.  Each
+// block stages the FP64 positions of its window (the tile plus one cell on
+// every side, tmd_window.hpp) from the cell-ordered mirror into LDS once; list entries are 16-bit window offsets read 8 at a time
+// (one int4 per lane, 1 KiB per wavefront), and the 8 neighbour positions
+// are read from LDS before any arithmetic.  The pair loop therefore streams
+// 2 bytes of list per entry from HBM and does no global gathers.
 //
 // Full lists accumulate the force of lane i in registers and store it once:
 // no atomics (core/compute.impala:92-123, full branch).
@@ -14,25 +19,22 @@
 // core/compute.impala:104-111).
 //
 // FP64 LJ follows the reference closure operation by operation
-// (core/runtime.impala:188-201), so with the reference's list order the
-// full-list FP64 force is bit-identical to the CPU restatement.
-// FP32-mixed: the distance test stays FP64 (identical in/out decisions at the
-// cutoff); the LJ algebra is FP32 and each component is accumulated with an
-// error-free TwoSum into a (sum, compensation) pair, combined in FP64.
+// (core/runtime.impala:188-201) in list order, so full-list FP64 forces are
+// bit-identical to the CPU restatement.  FP32-mixed: the distance test stays
+// FP64 (identical in/out decisions at the cutoff); the LJ algebra is FP32 and
+// each component is accumulated with an error-free TwoSum into a
+// (sum, compensation) pair, combined in FP64.
 #include "tmd_internal.hpp"
-#include "tmd_device.hpp"
+#include "tmd_window.hpp"
 
 namespace tmd {
 
 struct LJArgs {
     double rc2, rc, eps, sigma6, frc, urc;
-    float rc2f, epsf, sigma6f, frcf;
-    int cap, nwork, n;
+    float epsf, sigma6f, frcf;
+    int cap, nwork, n, wcap;
+    TileGeom g;
 };
-
-__device__ __forceinline__ const int* nb_column(const int* nbl, int w, int cap) {
-    return nbl + (size_t)(w >> 6) * (size_t)cap * 64 + (size_t)(w & 63);
-}
 
 // error-free transformation: s + x = t + e exactly (Knuth TwoSum)
 __device__ __forceinline__ void two_sum_acc(float& s, float& c, float x) {
@@ -62,78 +64,144 @@ __device__ __forceinline__ float lj_f32(const LJArgs& a, float rsq) {
     return f;
 }
 
-template <int SHIFT>
-__global__ void __launch_bounds__(256) k_lj_full_f64(const double4* __restrict__ pos, const int* __restrict__ work,
-                                                     const int* __restrict__ nbc, const int* __restrict__ nbl, LJArgs a,
-                                                     double4* __restrict__ force) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.nwork) return;
-    const int i = work[w];
-    const double4 pi = pos[i];
-    const int cnt = nbc[w];
-    const int* lp = nb_column(nbl, w, a.cap);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-#pragma unroll 4
-    for (int k = 0; k < cnt; ++k) {
-        const int j = lp[k * 64];
-        const double4 pj = pos[j];
-        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        if (rsq < a.rc2) {
-            const double f = lj_f64<SHIFT>(a, rsq);
-            fx = fx + dx * f;
-            fy = fy + dy * f;
-            fz = fz + dz * f;
-        }
-    }
-    force[i] = make_double4(fx, fy, fz, 0.0);
+struct Nb8 {
+    int t[8];
+};
+
+__device__ __forceinline__ Nb8 unpack8(const int4 q) {
+    Nb8 r;
+    r.t[0] = q.x & 0xffff; r.t[1] = (unsigned)q.x >> 16;
+    r.t[2] = q.y & 0xffff; r.t[3] = (unsigned)q.y >> 16;
+    r.t[4] = q.z & 0xffff; r.t[5] = (unsigned)q.z >> 16;
+    r.t[6] = q.w & 0xffff; r.t[7] = (unsigned)q.w >> 16;
+    return r;
 }
 
-template <int SHIFT>
-__global__ void __launch_bounds__(256) k_lj_full_f32(const double4* __restrict__ pos, const int* __restrict__ work,
-                                                     const int* __restrict__ nbc, const int* __restrict__ nbl, LJArgs a,
-                                                     double4* __restrict__ force) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.nwork) return;
+#define TMD_KERNEL_LIST_PARAMS                                                                                   \
+    const double4 *__restrict__ pos, const double4 *__restrict__ spos, const int *__restrict__ start,              \
+        const int *__restrict__ parts, const int *__restrict__ work, const int *__restrict__ tile_start,           \
+        const int *__restrict__ nbc, const nb_t *__restrict__ nbl
+
+template <int PREC, int SHIFT>
+__global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force) {
+    extern __shared__ double dyn[];
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
+    if (nloc == 0) return;
+    __shared__ TileWin W;
+    tile_window(tile, start, a.g, W);
+    double* sx = dyn;
+    double* sy = dyn + a.wcap;
+    double* sz = dyn + 2 * a.wcap;
+    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, nullptr);
+    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+    const int w = w0 + at;
     const int i = work[w];
     const double4 pi = pos[i];
     const int cnt = nbc[w];
-    const int* lp = nb_column(nbl, w, a.cap);
-    float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-#pragma unroll 4
-    for (int k = 0; k < cnt; ++k) {
-        const int j = lp[k * 64];
-        const double4 pj = pos[j];
-        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        if (rsq < a.rc2) {
-            const float f = lj_f32<SHIFT>(a, (float)rsq);
-            two_sum_acc(sx, cx, (float)dx * f);
-            two_sum_acc(sy, cy, (float)dy * f);
-            two_sum_acc(sz, cz, (float)dz * f);
+    const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
+    const int ng = (cnt + 7) >> 3;
+    for (int gi = 0; gi < ng; ++gi) {
+        const Nb8 e = unpack8(lp[(size_t)gi * 64]);
+        const int k0 = gi * 8;
+        double xs[8], ys[8], zs[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = (k0 + u < cnt) ? e.t[u] : 0;
+            if (staged) {
+                xs[u] = sx[t];
+                ys[u] = sy[t];
+                zs[u] = sz[t];
+            } else {
+                const double4 p = spos[win_index(t, W)];
+                xs[u] = p.x;
+                ys[u] = p.y;
+                zs[u] = p.z;
+            }
+        }
+        // branch-free: all 8 pair terms are evaluated (independent chains the
+        // scheduler can interleave), then accumulated in list order through
+        // selects, so the FP64 sum is the sequential one
+        bool ok[8];
+        double ddx[8], ddy[8], ddz[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ddx[u] = pi.x - xs[u];
+            ddy[u] = pi.y - ys[u];
+            ddz[u] = pi.z - zs[u];
+            const double rsq = ddx[u] * ddx[u] + ddy[u] * ddy[u] + ddz[u] * ddz[u];
+            ok[u] = (k0 + u < cnt) && (rsq < a.rc2);
+            xs[u] = ok[u] ? rsq : 1.0;  // reuse as the (safe) rsq
+        }
+        if (PREC == PREC_FP64) {
+            double ff[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ff[u] = lj_f64<SHIFT>(a, xs[u]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double nx = fx + ddx[u] * ff[u];
+                const double ny = fy + ddy[u] * ff[u];
+                const double nz = fz + ddz[u] * ff[u];
+                fx = ok[u] ? nx : fx;
+                fy = ok[u] ? ny : fy;
+                fz = ok[u] ? nz : fz;
+            }
+        } else {
+            float ff[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ff[u] = ok[u] ? lj_f32<SHIFT>(a, (float)xs[u]) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                two_sum_acc(sxa, cxa, (float)ddx[u] * ff[u]);
+                two_sum_acc(sya, cya, (float)ddy[u] * ff[u]);
+                two_sum_acc(sza, cza, (float)ddz[u] * ff[u]);
+            }
         }
     }
-    force[i] = make_double4((double)sx + (double)cx, (double)sy + (double)cy, (double)sz + (double)cz, 0.0);
+    if (PREC == PREC_FP64) force[i] = make_double4(fx, fy, fz, 0.0);
+    else force[i] = make_double4((double)sxa + (double)cxa, (double)sya + (double)cya, (double)sza + (double)cza, 0.0);
+    }
 }
 
 __device__ __forceinline__ unsigned long long to_fix(double v) { return (unsigned long long)__double2ll_rn(v * kFixScale); }
 
 template <int PREC, int SHIFT>
-__global__ void __launch_bounds__(256) k_lj_half(const double4* __restrict__ pos, const int* __restrict__ work,
-                                                 const int* __restrict__ nbc, const int* __restrict__ nbl, LJArgs a,
-                                                 unsigned long long* __restrict__ acc, int* __restrict__ flag) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.nwork) return;
+__global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS, LJArgs a, unsigned long long* __restrict__ acc,
+                                                         int* __restrict__ flag) {
+    extern __shared__ double dyn[];
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
+    if (nloc == 0) return;
+    __shared__ TileWin W;
+    tile_window(tile, start, a.g, W);
+    double* sx = dyn;
+    double* sy = dyn + a.wcap;
+    double* sz = dyn + 2 * a.wcap;
+    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
+    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+    const int w = w0 + at;
     const int i = work[w];
     const double4 pi = pos[i];
     const int cnt = nbc[w];
-    const int* lp = nb_column(nbl, w, a.cap);
     double fx = 0.0, fy = 0.0, fz = 0.0;
     bool big = false;
     for (int k = 0; k < cnt; ++k) {
-        const int j = lp[k * 64];
-        const double4 pj = pos[j];
-        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const int t = nbl[nb16_slot(w, k, a.cap)];
+        int j;
+        double xj, yj, zj;
+        if (staged) {
+            j = sj[t];
+            xj = sx[t]; yj = sy[t]; zj = sz[t];
+        } else {
+            const int kk = win_index(t, W);
+            j = parts[kk];
+            const double4 p = spos[kk];
+            xj = p.x; yj = p.y; zj = p.z;
+        }
+        const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
         const double rsq = dx * dx + dy * dy + dz * dz;
         if (rsq < a.rc2) {
             double Fx, Fy, Fz;
@@ -158,6 +226,7 @@ __global__ void __launch_bounds__(256) k_lj_half(const double4* __restrict__ pos
     atomicAdd(&acc[3 * (size_t)i + 1], to_fix(fy));
     atomicAdd(&acc[3 * (size_t)i + 2], to_fix(fz));
     if (big) atomicOr(flag, 1);
+    }
 }
 
 // fixed point -> FP64 forces; clears the accumulators for the next step
@@ -176,22 +245,49 @@ __global__ void k_fix_to_force(unsigned long long* __restrict__ acc, int n, doub
 // ---------------------------------------------------------------------------
 // energy and virial (always FP64): U = 4 eps (sr12 - sr6) [+ shift terms],
 // W = sum r.F; full lists weight each entry 1/2, half lists weight
-// local-local pairs 1 and local-ghost pairs 1/2.
+// local-local pairs 1 and local-ghost pairs 1/2.  Per-block partial sums in
+// a fixed order, reduced by one block in a fixed order: deterministic.
 template <int SHIFT>
-__global__ void __launch_bounds__(256) k_lj_energy(const double4* __restrict__ pos, const int* __restrict__ work,
-                                                   const int* __restrict__ nbc, const int* __restrict__ nbl, LJArgs a, int half,
-                                                   double* __restrict__ partials) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARAMS, LJArgs a, int half,
+                                                           double* __restrict__ partials) {
+    extern __shared__ double dyn[];
+    const int tile = blockIdx.x;
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
+    if (nloc == 0) {
+        if (threadIdx.x == 0) {
+            partials[3 * tile + 0] = 0.0;
+            partials[3 * tile + 1] = 0.0;
+            partials[3 * tile + 2] = 0.0;
+        }
+        return;
+    }
     double pe = 0.0, vir = 0.0, np = 0.0;
-    if (w < a.nwork) {
+    __shared__ TileWin W;
+    tile_window(tile, start, a.g, W);
+    double* sx = dyn;
+    double* sy = dyn + a.wcap;
+    double* sz = dyn + 2 * a.wcap;
+    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
+    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+        const int w = w0 + at;
         const int i = work[w];
         const double4 pi = pos[i];
         const int cnt = nbc[w];
-        const int* lp = nb_column(nbl, w, a.cap);
         for (int k = 0; k < cnt; ++k) {
-            const int j = lp[k * 64];
-            const double4 pj = pos[j];
-            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const int t = nbl[nb16_slot(w, k, a.cap)];
+            int j;
+            double xj, yj, zj;
+            if (staged) {
+                j = sj[t];
+                xj = sx[t]; yj = sy[t]; zj = sz[t];
+            } else {
+                const int kk = win_index(t, W);
+                j = parts[kk];
+                const double4 p = spos[kk];
+                xj = p.x; yj = p.y; zj = p.z;
+            }
+            const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
             const double rsq = dx * dx + dy * dy + dz * dz;
             if (rsq < a.rc2) {
                 const double sr2 = 1.0 / rsq;
@@ -210,9 +306,9 @@ __global__ void __launch_bounds__(256) k_lj_energy(const double4* __restrict__ p
             }
         }
     }
-    const double bpe = block_sum_double<256>(pe);
-    const double bvir = block_sum_double<256>(vir);
-    const double bnp = block_sum_double<256>(np);
+    const double bpe = block_sum_double<kTileThreads>(pe);
+    const double bvir = block_sum_double<kTileThreads>(vir);
+    const double bnp = block_sum_double<kTileThreads>(np);
     if (threadIdx.x == 0) {
         partials[3 * blockIdx.x + 0] = bpe;
         partials[3 * blockIdx.x + 1] = bvir;
@@ -220,7 +316,7 @@ __global__ void __launch_bounds__(256) k_lj_energy(const double4* __restrict__ p
     }
 }
 
-// single block, fixed order: sums `stride`-strided partial tuples of width k
+// single block, fixed order: column sums of `width`-wide partial tuples
 __global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partials, int nblocks, int width,
                                                          double* __restrict__ out) {
     for (int c = 0; c < width; ++c) {
@@ -242,7 +338,8 @@ void reduce_partials(Sim& s, const double* partials, int nblocks, int width, dou
 // F = (k delta - g_n v_n) n - min(g_t |v_t|, mu |k delta - g_n v_n|) v_t/|v_t|.
 struct DEMArgs {
     double rc2, d2, gn, gt, k, mu;
-    int cap, nwork, n;
+    int cap, nwork, n, wcap;
+    TileGeom g;
 };
 
 template <typename R>
@@ -266,23 +363,42 @@ __device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R r
 }
 
 template <int PREC, int HALF>
-__global__ void __launch_bounds__(256) k_dem(const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                             const int* __restrict__ work, const int* __restrict__ nbc,
-                                             const int* __restrict__ nbl, DEMArgs a, double4* __restrict__ force,
-                                             unsigned long long* __restrict__ acc, int* __restrict__ flag) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.nwork) return;
+__global__ void __launch_bounds__(kTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __restrict__ vel, DEMArgs a,
+                                                     double4* __restrict__ force, unsigned long long* __restrict__ acc,
+                                                     int* __restrict__ flag) {
+    extern __shared__ double dyn[];
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
+    if (nloc == 0) return;
+    __shared__ TileWin W;
+    tile_window(tile, start, a.g, W);
+    double* sx = dyn;
+    double* sy = dyn + a.wcap;
+    double* sz = dyn + 2 * a.wcap;
+    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
+    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+    const int w = w0 + at;
     const int i = work[w];
     const double4 pi = pos[i];
     const double4 vi = vel[i];
     const int cnt = nbc[w];
-    const int* lp = nb_column(nbl, w, a.cap);
     double fx = 0.0, fy = 0.0, fz = 0.0;
     bool big = false;
     for (int k = 0; k < cnt; ++k) {
-        const int j = lp[k * 64];
-        const double4 pj = pos[j];
-        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const int t = nbl[nb16_slot(w, k, a.cap)];
+        int j;
+        double xj, yj, zj;
+        if (staged) {
+            j = sj[t];
+            xj = sx[t]; yj = sy[t]; zj = sz[t];
+        } else {
+            const int kk = win_index(t, W);
+            j = parts[kk];
+            const double4 p = spos[kk];
+            xj = p.x; yj = p.y; zj = p.z;
+        }
+        const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
         const double rsq = dx * dx + dy * dy + dz * dz;
         if (rsq < a.rc2) {
             const double4 vj = vel[j];
@@ -313,6 +429,7 @@ __global__ void __launch_bounds__(256) k_dem(const double4* __restrict__ pos, co
     } else {
         force[i] = make_double4(fx, fy, fz, 0.0);
     }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -327,13 +444,14 @@ static LJArgs make_lj_args(const Sim& s, double rc, double eps, double sigma) {
     const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
     a.frc = 48.0 * sr6 * (sr6 - 0.5) * sr2 * eps * rc;
     a.urc = 4.0 * eps * sr6 * (sr6 - 1.0);
-    a.rc2f = (float)a.rc2;
     a.epsf = (float)eps;
     a.sigma6f = (float)a.sigma6;
     a.frcf = (float)a.frc;
     a.cap = s.nb_cap;
     a.nwork = s.n;
     a.n = s.n;
+    a.wcap = window_cap(s, true);
+    a.g = s.tiles;
     return a;
 }
 
@@ -353,46 +471,50 @@ static void check_fix_flag(Sim& s) {
     if (host) set_error(__FILE__, __LINE__, "half-list force exceeds the fixed-point accumulator range (|F| > 4e6)");
 }
 
+#define TMD_LIST_ARGS                                                                                              \
+    (const double4*)s.pos.as<double4>(), (const double4*)s.spos.as<double4>(), (const int*)s.cell_start.as<int>(), \
+        (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), (const int*)s.tile_start.as<int>(),     \
+        (const int*)s.nb_count.as<int>(), (const nb_t*)s.nb_list.as<nb_t>()
+
+static size_t lds_bytes(const Sim& s, bool with_index) {
+    return (size_t)window_cap(s, true) * (3 * sizeof(double) + (with_index ? sizeof(int) : 0));
+}
+
 void compute_lj(Sim& s, int half, double rc, double eps, double sigma) {
     if (has_error()) return;
     TMD_REQUIRE(s.nb_valid, "compute forces: no neighbour list (call md_assemble_neighborlists first)");
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
     if (has_error() || s.n == 0) return;
     LJArgs a = make_lj_args(s, rc, eps, sigma);
-    const int nt = 256, nb = div_up(s.n, nt);
+    const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     const int shift = s.lj_shift;
     hipEvent_t ev = nullptr;
+    s.timer.begin(K_MIRROR, s.stream, &ev);
+    refresh_mirror(s);
+    s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
     if (!half) {
-        const double4* P = s.pos.as<double4>();
-        const int* W = s.work.as<int>();
-        const int* C = s.nb_count.as<int>();
-        const int* L = s.nb_list.as<int>();
         double4* F = s.force.as<double4>();
         if (s.precision == PREC_FP64) {
-            if (shift) hipLaunchKernelGGL(k_lj_full_f64<1>, dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, F);
-            else hipLaunchKernelGGL(k_lj_full_f64<0>, dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, F);
+            if (shift) hipLaunchKernelGGL((k_lj_full<PREC_FP64, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
+            else hipLaunchKernelGGL((k_lj_full<PREC_FP64, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
         } else {
-            if (shift) hipLaunchKernelGGL(k_lj_full_f32<1>, dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, F);
-            else hipLaunchKernelGGL(k_lj_full_f32<0>, dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, F);
+            if (shift) hipLaunchKernelGGL((k_lj_full<PREC_FP32_MIXED, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
+            else hipLaunchKernelGGL((k_lj_full<PREC_FP32_MIXED, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
         }
     } else {
         ensure_acc(s);
         int* flag = s.counters.as<int>() + 4;
         TMD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s.stream));
-        const double4* P = s.pos.as<double4>();
-        const int* W = s.work.as<int>();
-        const int* C = s.nb_count.as<int>();
-        const int* L = s.nb_list.as<int>();
         unsigned long long* A = s.acc.as<unsigned long long>();
         if (s.precision == PREC_FP64) {
-            if (shift) hipLaunchKernelGGL((k_lj_half<PREC_FP64, 1>), dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, A, flag);
-            else hipLaunchKernelGGL((k_lj_half<PREC_FP64, 0>), dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, A, flag);
+            if (shift) hipLaunchKernelGGL((k_lj_half<PREC_FP64, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
+            else hipLaunchKernelGGL((k_lj_half<PREC_FP64, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
         } else {
-            if (shift) hipLaunchKernelGGL((k_lj_half<PREC_FP32_MIXED, 1>), dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, A, flag);
-            else hipLaunchKernelGGL((k_lj_half<PREC_FP32_MIXED, 0>), dim3(nb), dim3(nt), 0, s.stream, P, W, C, L, a, A, flag);
+            if (shift) hipLaunchKernelGGL((k_lj_half<PREC_FP32_MIXED, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
+            else hipLaunchKernelGGL((k_lj_half<PREC_FP32_MIXED, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
         }
-        hipLaunchKernelGGL(k_fix_to_force, dim3(nb), dim3(nt), 0, s.stream, A, s.n, s.force.as<double4>());
+        hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, s.force.as<double4>());
     }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
@@ -405,28 +527,28 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
     if (has_error() || s.n == 0) return;
     const double radius = s.dem_radius > 0.0 ? s.dem_radius : 0.5 * rc;
-    DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n};
-    const int nt = 256, nb = div_up(s.n, nt);
+    DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n, window_cap(s, true), s.tiles};
+    const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     ensure_acc(s);
     int* flag = s.counters.as<int>() + 4;
     TMD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s.stream));
-    const double4* P = s.pos.as<double4>();
     const double4* V = s.vel.as<double4>();
-    const int* W = s.work.as<int>();
-    const int* C = s.nb_count.as<int>();
-    const int* L = s.nb_list.as<int>();
     double4* F = s.force.as<double4>();
     unsigned long long* A = s.acc.as<unsigned long long>();
+    const size_t lds = lds_bytes(s, true);
     hipEvent_t ev = nullptr;
+    s.timer.begin(K_MIRROR, s.stream, &ev);
+    refresh_mirror(s);
+    s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
     if (s.precision == PREC_FP64) {
-        if (half) hipLaunchKernelGGL((k_dem<PREC_FP64, 1>), dim3(nb), dim3(nt), 0, s.stream, P, V, W, C, L, a, F, A, flag);
-        else hipLaunchKernelGGL((k_dem<PREC_FP64, 0>), dim3(nb), dim3(nt), 0, s.stream, P, V, W, C, L, a, F, A, flag);
+        if (half) hipLaunchKernelGGL((k_dem<PREC_FP64, 1>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        else hipLaunchKernelGGL((k_dem<PREC_FP64, 0>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
     } else {
-        if (half) hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 1>), dim3(nb), dim3(nt), 0, s.stream, P, V, W, C, L, a, F, A, flag);
-        else hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 0>), dim3(nb), dim3(nt), 0, s.stream, P, V, W, C, L, a, F, A, flag);
+        if (half) hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 1>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        else hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 0>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
     }
-    if (half) hipLaunchKernelGGL(k_fix_to_force, dim3(nb), dim3(nt), 0, s.stream, A, s.n, F);
+    if (half) hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F);
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
     if (half && s.sync_calls) check_fix_flag(s);
@@ -436,16 +558,15 @@ void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, d
     *pe = 0.0; *vir = 0.0; *pairs = 0;
     if (has_error() || !s.nb_valid || s.n == 0) return;
     LJArgs a = make_lj_args(s, rc, eps, sigma);
-    const int nt = 256, nb = div_up(s.n, nt);
+    const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    refresh_mirror(s);
     s.red.reserve(sizeof(double) * (3 * (size_t)nb + 8), false, s.stream);
     double* part = s.red.as<double>();
     double* out = part + 3 * (size_t)nb;
     if (s.lj_shift)
-        hipLaunchKernelGGL(k_lj_energy<1>, dim3(nb), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(), (const int*)s.work.as<int>(),
-                           (const int*)s.nb_count.as<int>(), (const int*)s.nb_list.as<int>(), a, s.nb_half, part);
+        hipLaunchKernelGGL(k_lj_energy<1>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
     else
-        hipLaunchKernelGGL(k_lj_energy<0>, dim3(nb), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(), (const int*)s.work.as<int>(),
-                           (const int*)s.nb_count.as<int>(), (const int*)s.nb_list.as<int>(), a, s.nb_half, part);
+        hipLaunchKernelGGL(k_lj_energy<0>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
     reduce_partials(s, part, nb, 3, out);
     double host[3] = {0, 0, 0};
     TMD_HIP(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, s.stream));

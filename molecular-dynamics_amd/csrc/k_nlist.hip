@@ -1,158 +1,184 @@
-// Verlet neighbour lists (full and half) from the cell grid.
+// Tile windows, the cell-ordered position mirror, and the Verlet neighbour
+// lists (full and half) as 16-bit offsets into each tile's window
+// (tmd_window.hpp).
 //
-// One 64-lane wavefront per cell.  The 27-cell stencil is 9 rows of 3 cells
-// that are contiguous in the cell CSR (cells are flattened x-fastest), so the
-// candidates of every atom of the cell are 9 contiguous ranges, staged once
-// into LDS (index + FP64 position).  The wave then walks the cell's atoms one
-// at a time and tests 64 candidates per pass, compacting the accepted ones
-// with a ballot so that every list keeps the reference's order: stencil
-// (z, y, x) then cell order (core/neighborlist.impala:1-78).  Distances are
-// computed in FP64 without contraction, so the sets are bit-exact with the
-// reference's `rsq <= r_nb^2` test.
-//
-// List layout ("tiled column-major"): work slot w (locals in cell order) owns
-// column (w & 63) of tile (w >> 6); entry k of w is at
-//   tile * cap * 64 + k * 64 + (w & 63)
-// so a wavefront of 64 consecutive slots reads 256 contiguous bytes per k.
+// List build: one block per tile, one lane per local atom of the tile.  The
+// block stages its window (atom index + FP64 position) into LDS; each lane
+// walks its 27-cell stencil as 9 rows of 3 cells inside the window, in the
+// reference's order -- stencil (z, y, x), then cell order
+// (core/neighborlist.impala:1-78) -- and keeps a candidate when
+// `rsq <= r_nb^2` in FP64 without contraction, so the sets are bit-exact with
+// the reference.  Half lists keep j < i among locals (the reference's
+// filter) plus, by default, every ghost.
 #include "tmd_internal.hpp"
-#include "tmd_device.hpp"
 
 namespace tmd {
 
-constexpr int kNbStage = 768;
-
-struct NbArgs {
-    int n, nx, ny, ncells, cap, half, half_mode;
-    double cut2;
-};
-
-__device__ __forceinline__ size_t nb_index(int w, int k, int cap) {
-    return (size_t)(w >> 6) * (size_t)cap * 64 + (size_t)k * 64 + (size_t)(w & 63);
+// Cell-ordered position mirror: spos[k] = pos[parts[k]] (locals and ghosts).
+__global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict__ parts, const int* __restrict__ nbinned,
+                         double4* __restrict__ spos) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *nbinned) return;
+    spos[k] = pos[parts[k]];
 }
 
-__global__ void __launch_bounds__(64) k_nlist(const double4* __restrict__ pos, const int* __restrict__ start,
-                                               const int* __restrict__ parts, const int* __restrict__ work_start,
-                                               NbArgs a, int* __restrict__ nb_count, int* __restrict__ nb_list,
-                                               int* __restrict__ overflow) {
-    const int c = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int wbeg = work_start[c];
-    const int nloc = work_start[c + 1] - wbeg;
+struct NbArgs {
+    int n, cap, half, half_mode, wcap;
+    double cut2;
+    TileGeom g;
+};
+
+__global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restrict__ spos, const double4* __restrict__ pos,
+                                                        const int* __restrict__ cell_of, const int* __restrict__ start,
+                                                        const int* __restrict__ parts, const int* __restrict__ work,
+                                                        const int* __restrict__ tile_start, NbArgs a,
+                                                        int* __restrict__ nb_count, nb_t* __restrict__ nb_list,
+                                                        int* __restrict__ overflow) {
+    extern __shared__ double dyn[];
+    const int tile = blockIdx.x;
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
+    __shared__ TileWin W;
+    tile_window(tile, start, a.g, W);
+    double* sx = dyn;
+    double* sy = dyn + a.wcap;
+    double* sz = dyn + 2 * a.wcap;
+    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
+    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
 
-    __shared__ int rbeg[9], roff[10];
-    __shared__ int lj[kNbStage];
-    __shared__ double lx[kNbStage], ly[kNbStage], lz[kNbStage];
-
-    if (lane < 9) {
-        const int oz = lane / 3 - 1, oy = lane % 3 - 1;
-        const int mid = c + oz * a.nx * a.ny + oy * a.nx;
-        const int lo = max(mid - 1, 0);
-        const int hi = min(mid + 1, a.ncells - 1);
-        int b = 0, len = 0;
-        if (lo <= hi) {
-            b = start[lo];
-            len = start[hi + 1] - b;
-        }
-        rbeg[lane] = b;
-        roff[lane + 1] = len;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        roff[0] = 0;
-        for (int r = 0; r < 9; ++r) roff[r + 1] += roff[r];
-    }
-    __syncthreads();
-    const int S = roff[9];
-    const bool staged = S <= kNbStage;
-    if (staged) {
-        for (int t = lane; t < S; t += 64) {
-            int r = 0;
-            while (t >= roff[r + 1]) ++r;
-            const int j = parts[rbeg[r] + (t - roff[r])];
-            const double4 p = pos[j];
-            lj[t] = j;
-            lx[t] = p.x;
-            ly[t] = p.y;
-            lz[t] = p.z;
-        }
-    }
-    __syncthreads();
-
-    const int cbeg = start[c];
-    for (int at = 0; at < nloc; ++at) {
-        const int w = wbeg + at;
-        const int i = parts[cbeg + at];
+    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+        const int w = w0 + at;
+        const int i = work[w];
+        const int c = cell_of[i];
+        const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
         const double4 pi = pos[i];
         int cnt = 0;
-        for (int base = 0; base < S; base += 64) {
-            const int t = base + lane;
-            bool ok = false;
-            int j = -1;
-            if (t < S) {
-                double xj, yj, zj;
-                if (staged) {
-                    j = lj[t];
-                    xj = lx[t]; yj = ly[t]; zj = lz[t];
-                } else {
-                    int r = 0;
-                    while (t >= roff[r + 1]) ++r;
-                    j = parts[rbeg[r] + (t - roff[r])];
-                    const double4 p = pos[j];
-                    xj = p.x; yj = p.y; zj = p.z;
-                }
-                bool keep = (j != i);
-                if (keep && a.half) keep = (a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j);
-                if (keep) {
+        for (int r = 0; r < 9; ++r) {
+            int t0, t1;
+            win_stencil_row(cx, cy, cz, r % 3 - 1, r / 3 - 1, start, a.g, W, t0, t1);
+            // 4 candidates per pass: loads and distances first (independent),
+            // then in-order appends
+            for (int tb = t0; tb < t1; tb += 4) {
+                int jj[4];
+                bool ok[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = min(tb + u, t1 - 1);
+                    double xj, yj, zj;
+                    if (staged) {
+                        jj[u] = sj[t];
+                        xj = sx[t]; yj = sy[t]; zj = sz[t];
+                    } else {
+                        const int kk = win_index(t, W);
+                        jj[u] = parts[kk];
+                        const double4 p = spos[kk];
+                        xj = p.x; yj = p.y; zj = p.z;
+                    }
+                    const int j = jj[u];
+                    bool keep = (tb + u < t1) && (j != i);
+                    if (a.half) keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
                     const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                     const double rsq = dx * dx + dy * dy + dz * dz;
-                    ok = rsq <= a.cut2;
+                    ok[u] = keep && rsq <= a.cut2;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (ok[u] && cnt < a.cap) nb_list[nb16_slot(w, cnt, a.cap)] = (nb_t)(tb + u);
+                    cnt += ok[u] ? 1 : 0;
                 }
             }
-            const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int idx = cnt + __popcll(m & ((1ull << lane) - 1ull));
-                if (idx < a.cap) nb_list[nb_index(w, idx, a.cap)] = j;
-            }
-            cnt += __popcll(m);
         }
-        if (lane == 0) {
-            nb_count[w] = cnt;
-            if (cnt > a.cap) atomicMax(overflow, cnt);
-        }
+        nb_count[w] = cnt;
+        if (cnt > a.cap) atomicMax(overflow, cnt);
     }
+}
+
+// ---------------------------------------------------------------------------
+int window_cap(const Sim& s, bool with_index) {
+    const int per = 3 * (int)sizeof(double) + (with_index ? (int)sizeof(int) : 0);
+    return std::min(s.max_window, kLdsBudget / per);
+}
+
+void build_windows(Sim& s) {
+    if (has_error()) return;
+    int* counters = s.counters.as<int>();
+    const Geom& G = s.g;
+    const int ntiles = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    // counters[6] = largest tile window (from k_tile_stats)
+    int host[3] = {0, 0, 0};
+    TMD_HIP(hipMemcpyAsync(&host[0], s.tile_start.as<int>() + ntiles, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipMemcpyAsync(&host[1], s.cell_start.as<int>() + G.ncells, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipMemcpyAsync(&host[2], counters + 6, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipStreamSynchronize(s.stream));
+    if (has_error()) return;
+    if (host[0] != s.n) {
+        char msg[200];
+        snprintf(msg, sizeof msg,
+                 "%d local atoms lie outside the periodic box or the cell grid (non-finite or unwrapped positions; "
+                 "md_enforce_pbc wraps them)",
+                 s.n - host[0]);
+        set_error(__FILE__, __LINE__, msg);
+        return;
+    }
+    s.nbinned = host[1];
+    s.max_window = host[2];
+    s.spos.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+    if (has_error()) return;
+    if (s.max_window > kWinMax) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "a tile neighbourhood holds %d atoms, beyond the 16-bit window range (%d)", s.max_window,
+                 kWinMax);
+        set_error(__FILE__, __LINE__, msg);
+        return;
+    }
+    if (kLdsBudget > s.lds_limit) {
+        set_error(__FILE__, __LINE__, "device LDS per block is below the window budget");
+        return;
+    }
+    s.windows_valid = 1;
+}
+
+void refresh_mirror(Sim& s) {
+    if (has_error() || s.nbinned == 0) return;
+    const int nt = 256;
+    hipLaunchKernelGGL(k_mirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
+                       (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
+                       s.spos.as<double4>());
+    TMD_HIP(hipGetLastError());
 }
 
 void build_neighbor_lists(Sim& s, int half, double cutoff) {
     if (has_error()) return;
-    const Geom& G = s.g;
+    TMD_REQUIRE(s.windows_valid, "md_assemble_neighborlists: cells not binned (call md_distribute_particles first)");
+    if (has_error()) return;
     int* counters = s.counters.as<int>();
-    // counters[2] = overflow, counters[3] = number of work slots (binned locals)
+    refresh_mirror(s);
+    const int ntiles = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    // counters[2] = overflow (largest count above the capacity)
     for (;;) {
-        const int ntile = div_up(s.n > 0 ? s.n : 1, 64);
-        s.nb_stride = ntile * 64;
+        s.nb_cap = (s.nb_cap + 7) & ~7;
+        const int ntile64 = div_up(s.n > 0 ? s.n : 1, 64);
+        s.nb_stride = ntile64 * 64;
         s.nb_count.reserve(sizeof(int) * (size_t)s.nb_stride, false, s.stream);
-        s.nb_list.reserve(sizeof(int) * (size_t)ntile * 64 * (size_t)s.nb_cap, false, s.stream);
+        s.nb_list.reserve(sizeof(nb_t) * (size_t)ntile64 * 64 * (size_t)s.nb_cap, false, s.stream);
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
-        NbArgs a{s.n, G.nx, G.ny, G.ncells, s.nb_cap, half, s.half_mode, cutoff * cutoff};
-        hipLaunchKernelGGL(k_nlist, dim3(G.ncells), dim3(64), 0, s.stream, (const double4*)s.pos.as<double4>(),
-                           (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(),
-                           (const int*)s.work_start.as<int>(), a, s.nb_count.as<int>(), s.nb_list.as<int>(), counters + 2);
+        const int wcap = window_cap(s, true);
+        NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
+        const size_t lds = (size_t)wcap * (3 * sizeof(double) + sizeof(int));
+        if (s.n > 0)
+            hipLaunchKernelGGL(k_nlist, dim3(ntiles), dim3(kTileThreads), lds, s.stream, (const double4*)s.spos.as<double4>(),
+                               (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(),
+                               (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(),
+                               (const int*)s.work.as<int>(), (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(),
+                               s.nb_list.as<nb_t>(), counters + 2);
         TMD_HIP(hipGetLastError());
-        int host[2] = {0, 0};
-        TMD_HIP(hipMemcpyAsync(&host[0], counters + 2, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-        TMD_HIP(hipMemcpyAsync(&host[1], s.work_start.as<int>() + G.ncells, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+        int over = 0;
+        TMD_HIP(hipMemcpyAsync(&over, counters + 2, sizeof(int), hipMemcpyDeviceToHost, s.stream));
         TMD_HIP(hipStreamSynchronize(s.stream));
         if (has_error()) return;
-        if (host[1] != s.n) {
-            char msg[160];
-            snprintf(msg, sizeof msg, "%d local atoms lie outside the cell grid (non-finite or escaped positions)", s.n - host[1]);
-            set_error(__FILE__, __LINE__, msg);
-            return;
-        }
-        if (host[0] > 0) {
-            s.nb_cap = host[0] * 2;
+        if (over > 0) {
+            s.nb_cap = over * 2;
             continue;
         }
         break;

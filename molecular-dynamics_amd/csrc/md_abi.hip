@@ -118,6 +118,8 @@ static bool ensure_stream(Sim& s) {
     }
     TMD_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     if (has_error()) return false;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) == hipSuccess && lds > 0) s.lds_limit = lds;
     s.counters.reserve(64 * sizeof(int), false, s.stream);
     TMD_HIP(hipMemsetAsync(s.counters.p, 0, 64 * sizeof(int), s.stream));
     return !has_error();
@@ -417,12 +419,15 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.work_start, &s.work, &s.nb_count, &s.nb_list,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.work_start, &s.work, &s.tile_start, &s.spos,
+                      &s.nb_count, &s.nb_list,
                       &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf};
     for (DevBuf* b : bufs) b->release();
     s.n = s.ng = 0;
     s.cap_all = 0;
     s.nb_valid = 0;
+    s.windows_valid = 0;
+    s.nbinned = 0;
     s.dev_valid = false;
 }
 
@@ -494,6 +499,7 @@ void md_borders() {
     build_ghosts(s);
     s.timer.end(K_BORDERS, s.stream, ev);
     s.nb_valid = 0;
+    s.windows_valid = 0;
     finish(s);
 }
 
@@ -501,6 +507,7 @@ void md_exchange_particles() {
     Sim& s = sim();
     s.ng = 0;
     s.nb_valid = 0;
+    s.windows_valid = 0;
 }
 
 double md_get_send_buffer_value(int) { return 0.0; }
@@ -594,6 +601,15 @@ void tmd_set_sync(int on) { sim().sync_calls = on ? 1 : 0; }
 void tmd_set_dem_radius(double r) { sim().dem_radius = r; }
 void tmd_set_lj_shift(int on) { sim().lj_shift = on ? 1 : 0; }
 
+void tmd_set_tile(int tx, int ty, int tz) {
+    Sim& s = sim();
+    s.tile_req[0] = tx;
+    s.tile_req[1] = ty;
+    s.tile_req[2] = tz;
+    s.windows_valid = 0;
+    s.nb_valid = 0;
+}
+
 void tmd_thermo(double rc, double eps, double sigma, double* out) {
     Sim& s = sim();
     for (int k = 0; k < 6; ++k) out[k] = 0.0;
@@ -654,21 +670,55 @@ int tmd_get_cells(int* start, int* atoms, int* atom_cell) {
 }
 
 int tmd_get_neighbor_lists(int* counts, int* list, int stride) {
+    // lists hold 16-bit offsets into each tile's window; resolve them to atom
+    // indices with the same window construction as the kernels
     Sim& s = sim();
     if (!s.stream || has_error() || !s.nb_valid) return 0;
-    const int n = s.n, cap = s.nb_cap, ntile = (n + 63) / 64;
-    std::vector<int> work(std::max(n, 1)), cnt(std::max(n, 1));
-    std::vector<int> lst((size_t)ntile * 64 * cap);
+    const int n = s.n, cap = s.nb_cap, ntile64 = (n + 63) / 64, nc = s.g.ncells;
+    const TileGeom g = s.tiles;
+    const int ntiles = g.ntx * g.nty * g.ntz;
+    std::vector<int> work(std::max(n, 1)), cnt(std::max(n, 1)), st(nc + 1), ts(ntiles + 1);
     TMD_HIP(hipMemcpy(work.data(), s.work.p, sizeof(int) * n, hipMemcpyDeviceToHost));
     TMD_HIP(hipMemcpy(cnt.data(), s.nb_count.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    TMD_HIP(hipMemcpy(lst.data(), s.nb_list.p, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+    TMD_HIP(hipMemcpy(st.data(), s.cell_start.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost));
+    TMD_HIP(hipMemcpy(ts.data(), s.tile_start.p, sizeof(int) * (ntiles + 1), hipMemcpyDeviceToHost));
+    std::vector<int> parts(std::max(st[nc], 1));
+    TMD_HIP(hipMemcpy(parts.data(), s.cell_parts.p, sizeof(int) * st[nc], hipMemcpyDeviceToHost));
+    std::vector<unsigned short> lst(list ? (size_t)ntile64 * 64 * cap : 1);
+    if (list) TMD_HIP(hipMemcpy(lst.data(), s.nb_list.p, sizeof(unsigned short) * lst.size(), hipMemcpyDeviceToHost));
     if (has_error()) return 0;
-    for (int w = 0; w < n; ++w) {
-        const int i = work[w];
-        counts[i] = cnt[w];
-        if (list) {
+    std::vector<int> seg_beg, seg_off;
+    for (int tile = 0; tile < ntiles; ++tile) {
+        const int w0 = ts[tile], w1 = ts[tile + 1];
+        if (w0 == w1) continue;
+        const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
+        const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+        const int wy = g.ty + 2, nseg = wy * (g.tz + 2);
+        seg_beg.assign(nseg, 0);
+        seg_off.assign(nseg + 1, 0);
+        for (int q = 0; q < nseg; ++q) {
+            const int y = cy0 - 1 + q % wy, z = cz0 - 1 + q / wy;
+            const int xlo = cx0 - 1, xhi = std::min(cx0 + g.tx, g.nx - 1);
+            int b = 0, len = 0;
+            if (y < g.ny && z < g.nz) {
+                const int row = (z * g.ny + y) * g.nx;
+                b = st[row + xlo];
+                len = st[row + xhi + 1] - b;
+            }
+            seg_beg[q] = b;
+            seg_off[q + 1] = seg_off[q] + len;
+        }
+        for (int w = w0; w < w1; ++w) {
+            const int i = work[w];
+            counts[i] = cnt[w];
+            if (!list) continue;
             const int m = std::min(std::min(cnt[w], stride), cap);
-            for (int k = 0; k < m; ++k) list[(size_t)i * stride + k] = lst[(size_t)(w >> 6) * cap * 64 + (size_t)k * 64 + (w & 63)];
+            for (int k = 0; k < m; ++k) {
+                const int t = lst[(size_t)(w >> 6) * cap * 64 + (size_t)(k >> 3) * 512 + (size_t)(w & 63) * 8 + (size_t)(k & 7)];
+                int q = 0;
+                while (q + 1 < nseg && seg_off[q + 1] <= t) ++q;
+                list[(size_t)i * stride + k] = parts[seg_beg[q] + (t - seg_off[q])];
+            }
         }
     }
     return n;

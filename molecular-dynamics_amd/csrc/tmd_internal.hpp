@@ -3,10 +3,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
 #include <vector>
+
+#include "tmd_window.hpp"
 
 namespace tmd {
 
@@ -64,7 +67,7 @@ struct DevBuf {
 // Per-kernel timing with HIP events on the library stream.
 enum KernelId {
     K_FORCE = 0, K_INTEGRATE_1, K_INTEGRATE_2, K_GHOST_SYNC, K_NEIGHBOR, K_BIN, K_BORDERS,
-    K_THERMO, K_ADJOINT, K_NKERNELS
+    K_THERMO, K_ADJOINT, K_MIRROR, K_NKERNELS
 };
 struct KernelTimer {
     bool enabled = false;
@@ -88,6 +91,12 @@ struct Sim {
     int nb_stride = 0;  // column stride of the list (multiple of 64 >= n)
     int nb_half = 0;    // list built as a half list
     int nb_valid = 0;   // list present
+    int nbinned = 0;    // atoms in the cell CSR (locals + ghosts inside the grid)
+    int max_window = 0; // largest tile window (atoms)
+    TileGeom tiles{};   // tile decomposition of the cell grid
+    int tile_req[3] = {0, 0, 0};  // requested cells per tile (0: automatic)
+    int windows_valid = 0;
+    int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
     int half_mode = HALF_GHOST_FULL;
@@ -106,7 +115,9 @@ struct Sim {
     DevBuf pos, vel, force, mass;      // double4 x3, double
     DevBuf g_owner, g_src, g_code;     // ghost table
     DevBuf cell_of, cell_slot, cell_count, cell_start, cell_parts;
-    DevBuf work_start, work;           // locals in cell order (work slots)
+    DevBuf work_start, work;           // per-tile local counts; locals in tile work order (work slots)
+    DevBuf tile_start;                 // first work slot of each tile [ntiles + 1]
+    DevBuf spos;                       // cell-ordered position mirror (double4)
     DevBuf nb_count, nb_list;
     DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
@@ -139,7 +150,10 @@ void refresh_ghosts(Sim& s);
 // binning
 void bin_particles(Sim& s);
 
-// neighbour lists
+// tile windows (after binning), position mirror, neighbour lists
+void build_windows(Sim& s);
+int window_cap(const Sim& s, bool with_index);  // staged atoms per block (LDS budget)
+void refresh_mirror(Sim& s);
 void build_neighbor_lists(Sim& s, int half, double cutoff);
 
 // forces

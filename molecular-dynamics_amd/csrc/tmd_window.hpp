@@ -1,0 +1,132 @@
+// Tile windows: LDS staging of the neighbourhood of a tile of cells.
+//
+// The cells that can hold local atoms (1..n-2 in each dimension) are cut
+// into tiles of TX x TY x TZ cells.  The work order
+// (local atoms in the order the list and force kernels walk them) is
+// tile-major: tiles in (z, y, x) order, the cells of a tile in (z, y, x)
+// order, the locals of a cell in index order.  One block handles one tile.
+// The window of a tile is the box of cells one wider on every side; each of
+// its (TY+2)(TZ+2) x-rows is contiguous in the cell CSR, so the window is the
+// concatenation of those CSR ranges, staged into LDS once per block from
+// the cell-ordered position mirror (contiguous loads).  A list entry is a
+// 16-bit offset into the window.  An atom's 27-cell stencil is 9 rows of 3
+// cells, each inside one window row, so lists keep the reference's stencil
+// order (core/neighborlist.impala:1-16) and cell order within each cell.
+//
+// Locals always lie in cells 1..n-2 of each dimension (the grid is the box
+// extended by one cell on each side), so their stencils never wrap and
+// always lie inside their tile's window.
+#pragma once
+
+#include "tmd_device.hpp"
+
+namespace tmd {
+
+constexpr int kTileThreads = 512;  // threads per tile block
+constexpr int kMaxTile = 8;        // cells per tile dimension, at most
+constexpr int kMaxSeg = (kMaxTile + 2) * (kMaxTile + 2);
+constexpr int kWinMax = 65535;     // window offsets are 16-bit
+constexpr int kLdsBudget = 80 * 1024;  // LDS per block: two blocks per 160 KiB CU
+
+typedef unsigned short nb_t;
+
+// u16 list layout: tiles of 64 work slots, entry k of column l at
+// (k / 8) * 512 + l * 8 + (k % 8); one int4 load reads 8 entries of a lane.
+__device__ __forceinline__ size_t nb16_slot(int w, int k, int cap) {
+    return (size_t)(w >> 6) * (size_t)cap * 64 + (size_t)(k >> 3) * 512 + (size_t)(w & 63) * 8 + (size_t)(k & 7);
+}
+
+struct TileGeom {
+    int nx, ny, nz;     // cells
+    int tx, ty, tz;     // cells per tile
+    int ntx, nty, ntz;  // tiles
+};
+
+struct TileWin {
+    int cx0, cy0, cz0;          // first cell of the tile
+    int nseg;                   // (ty + 2) * (tz + 2)
+    int seg_beg[kMaxSeg];       // CSR start of each window row
+    int seg_off[kMaxSeg + 1];   // window offset of each row (prefix)
+};
+
+// Build the tile's window (whole block; barriers inside).  Row s covers
+// cells x in [cx0-1, cx0+tx] (clipped) at y = cy0-1 + s % (ty+2),
+// z = cz0-1 + s / (ty+2).
+__device__ __forceinline__ void tile_window(int tile, const int* __restrict__ start, TileGeom g, TileWin& W) {
+    const int t = threadIdx.x;
+    const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
+    const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+    const int wy = g.ty + 2;
+    const int nseg = wy * (g.tz + 2);
+    if (t < nseg) {
+        const int y = cy0 - 1 + t % wy, z = cz0 - 1 + t / wy;
+        const int xlo = max(cx0 - 1, 0), xhi = min(cx0 + g.tx, g.nx - 1);
+        int b = 0, len = 0;
+        if (y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
+            const int row = (z * g.ny + y) * g.nx;
+            b = start[row + xlo];
+            len = start[row + xhi + 1] - b;
+        }
+        W.seg_beg[t] = b;
+        W.seg_off[t + 1] = len;
+    }
+    if (t == 0) {
+        W.cx0 = cx0;
+        W.cy0 = cy0;
+        W.cz0 = cz0;
+        W.nseg = nseg;
+    }
+    __syncthreads();
+    if (t == 0) {
+        W.seg_off[0] = 0;
+        for (int s = 0; s < nseg; ++s) W.seg_off[s + 1] += W.seg_off[s];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int win_size(const TileWin& W) { return W.seg_off[W.nseg]; }
+
+// CSR index (mirror index) of window offset t: binary search over the rows
+__device__ __forceinline__ int win_index(int t, const TileWin& W) {
+    int lo = 0, hi = W.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (W.seg_off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return W.seg_beg[lo] + (t - W.seg_off[lo]);
+}
+
+// Window offsets [t0, t1) of the 3 cells x-1..x+1 at stencil row (oy, oz)
+// of an atom in cell (cx, cy, cz) of this tile.
+__device__ __forceinline__ void win_stencil_row(int cx, int cy, int cz, int oy, int oz, const int* __restrict__ start,
+                                                TileGeom g, const TileWin& W, int& t0, int& t1) {
+    const int y = cy + oy, z = cz + oz;
+    const int s = (z - (W.cz0 - 1)) * (g.ty + 2) + (y - (W.cy0 - 1));
+    const int row = (z * g.ny + y) * g.nx;
+    const int kb = start[row + cx - 1], ke = start[row + cx + 2];
+    t0 = W.seg_off[s] + (kb - W.seg_beg[s]);
+    t1 = t0 + (ke - kb);
+}
+
+// Stage the window's FP64 positions and, when sj != nullptr, atom indices
+// into LDS if it fits wcap atoms; otherwise consumers read the mirror.
+__device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const double4* __restrict__ spos,
+                                           const int* __restrict__ parts, double* sx, double* sy, double* sz, int* sj) {
+    const int S = win_size(W);
+    const bool staged = S <= wcap;
+    if (staged) {
+        for (int t = threadIdx.x; t < S; t += blockDim.x) {
+            const int k = win_index(t, W);
+            const double4 p = spos[k];
+            sx[t] = p.x;
+            sy[t] = p.y;
+            sz[t] = p.z;
+            if (sj) sj[t] = parts[k];
+        }
+    }
+    __syncthreads();
+    return staged;
+}
+
+}  // namespace tmd

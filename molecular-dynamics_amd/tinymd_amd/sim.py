@@ -13,7 +13,7 @@ PRECISION = {"fp64": 0, "fp32mixed": 1, "fp32": 1}
 HALF_MODE = {"ghost_full": 0, "ref_literal": 1}
 GHOST_MODE = {"fresh": 0, "ref_lag": 1}
 KERNELS = {"force": 0, "integrate1": 1, "integrate2": 2, "ghost_sync": 3, "neighbor": 4, "bin": 5,
-           "borders": 6, "thermo": 7, "adjoint": 8}
+           "borders": 6, "thermo": 7, "adjoint": 8, "mirror": 9}
 
 
 def _dptr(a):

@@ -255,3 +255,23 @@ def test_full_size_lattice_invariants(tm):
     assert abs(th["PE"] / sim.nlocal - (-6.7733178)) < 5e-7
     assert abs(th["T"] - 1.44) < 1e-12
     sim.close()
+
+
+def test_trajectory_bitwise_16cube_staged_windows(tm):
+    """16^3 (16,384 atoms): large enough that block windows are staged in
+    LDS; FP64 trajectory must stay bit-identical to the oracle."""
+    announce("trajectory 16^3, 45 steps, FP64 full, bit-exact")
+    aabb, m, x, v = perturbed_fcc(16, amp=0.05, seed=21)
+    sim, orc = make_pair(aabb, m, x, v, threads=8)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    sim.run_lj(45, reneigh=20, dt=0.005)
+    orc.run_lj(45, reneigh=20, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert np.array_equal(cg, co)
+    assert all(np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]) for i in range(0, len(cg), 97))
