@@ -20,8 +20,9 @@ int tmd_device_info(char* buf, int len);
 
 /* ---- modes ---------------------------------------------------------------- */
 /* 0 = FP64 (default; the reference's real_t, utils/utilities.impala:1),
- * 1 = FP32-mixed pair forces with compensated accumulation (FP64 positions,
- *     velocities, wrap, ghosts and reductions). */
+ * 1 = FP32-mixed: FP32 pair forces accumulated per atom in FP64 (FP64
+ *     positions, distance test, velocities, wrap, ghosts and reductions),
+ * 2 = as 1 but accumulated in FP32 with Kahan compensation (full lists). */
 void tmd_set_precision(int mode);
 /* Half-list filter.  0 = local j < i plus every ghost (each periodic pair
  * once; default), 1 = the reference's literal i > j filter, which drops all

@@ -22,8 +22,9 @@
 // (core/runtime.impala:188-201) in list order, so full-list FP64 forces are
 // bit-identical to the CPU restatement.  FP32-mixed: the distance test stays
 // FP64 (identical in/out decisions at the cutoff); the LJ algebra is FP32 and
-// each component is accumulated with an error-free TwoSum into a
-// (sum, compensation) pair, combined in FP64.
+// dx * f is accumulated in FP64 (each FP32 term is exact there, and the
+// extra mantissa bits absorb the summation rounding).  The Kahan variant
+// accumulates in FP32 with a compensation term instead.
 #include "tmd_internal.hpp"
 #include "tmd_window.hpp"
 
@@ -82,7 +83,118 @@ __device__ __forceinline__ Nb8 unpack8(const int4 q) {
         const int *__restrict__ parts, const int *__restrict__ work, const int *__restrict__ tile_start,           \
         const int *__restrict__ nbc, const nb_t *__restrict__ nbl
 
-template <int PREC, int SHIFT>
+// FP64 pair scalar of the reference closure; UNIT skips the exact
+// multiplications by eps = 1 and sigma^6 = 1 (x * 1.0 == x), so the result
+// is bit-identical either way.
+template <int SHIFT, int UNIT>
+__device__ __forceinline__ double lj_f64u(const LJArgs& a, double rsq) {
+    const double sr2 = 1.0 / rsq;
+    const double sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6;
+    double f = UNIT ? 48.0 * sr6 * (sr6 - 0.5) * sr2 : 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
+    if (SHIFT) f = f - a.frc / sqrt(rsq);
+    return f;
+}
+
+template <int SHIFT, int UNIT>
+__device__ __forceinline__ float lj_f32u(const LJArgs& a, float rsq) {
+    const float sr2 = __builtin_amdgcn_rcpf(rsq);
+    const float sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6f;
+    float f = UNIT ? 48.0f * sr6 * (sr6 - 0.5f) * sr2 : 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
+    if (SHIFT) f = f - a.frcf * __builtin_amdgcn_rsqf(rsq);
+    return f;
+}
+
+// Kahan compensated summation (s, c) += x
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Neighbour position of window offset t: LDS (x, y, z interleaved) when the
+// window is staged, else the cell-ordered mirror.
+template <bool STAGED>
+__device__ __forceinline__ void win_pos(int t, const double* __restrict__ sp, const double4* __restrict__ spos, const TileWin& W,
+                                        double& x, double& y, double& z) {
+    if (STAGED) {
+        x = sp[3 * t + 0];
+        y = sp[3 * t + 1];
+        z = sp[3 * t + 2];
+    } else {
+        const double4 p = spos[win_index(t, W)];
+        x = p.x;
+        y = p.y;
+        z = p.z;
+    }
+}
+
+// Full-list force on one atom (lane).  Entries are read 8 at a time; the 8
+// pair terms are evaluated without branches (masked entries use a safe
+// rsq and a zero scalar) and accumulated in list order.  Adding a masked
+// term adds dx * 0.0 = +-0.0, which leaves the running sum unchanged (the
+// sum starts at +0.0 and never becomes -0.0), so the FP64 result equals the
+// sequential reference sum bit for bit.
+template <int PREC, int SHIFT, int UNIT, bool STAGED>
+__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const double* __restrict__ sp,
+                                                const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
+    const int ng = (cnt + 7) >> 3;
+    for (int gi = 0; gi < ng; ++gi) {
+        const Nb8 e = unpack8(lp[(size_t)gi * 64]);
+        const int k0 = gi * 8;
+        double dx[8], dy[8], dz[8], rs[8];
+        bool ok[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool valid = k0 + u < cnt;
+            double xj, yj, zj;
+            win_pos<STAGED>(valid ? e.t[u] : 0, sp, spos, W, xj, yj, zj);
+            dx[u] = pi.x - xj;
+            dy[u] = pi.y - yj;
+            dz[u] = pi.z - zj;
+            const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            ok[u] = valid && rsq < a.rc2;
+            rs[u] = ok[u] ? rsq : 1.0;
+        }
+        if (PREC == PREC_FP64) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double f = lj_f64u<SHIFT, UNIT>(a, rs[u]);
+                const double fm = ok[u] ? f : 0.0;
+                fx = fx + dx[u] * fm;
+                fy = fy + dy[u] * fm;
+                fz = fz + dz[u] * fm;
+            }
+        } else if (PREC == PREC_FP32_MIXED) {
+            // FP32 pair scalar, FP64 accumulation of dx * f (each FP32 term is
+            // exact in FP64; the 29 extra mantissa bits absorb the sum's rounding)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
+                const double fm = (double)(ok[u] ? f : 0.0f);
+                fx = fx + dx[u] * fm;
+                fy = fy + dy[u] * fm;
+                fz = fz + dz[u] * fm;
+            }
+        } else {
+            // FP32 pair terms with Kahan-compensated FP32 accumulation
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
+                const float fm = ok[u] ? f : 0.0f;
+                kahan_add(sxa, cxa, (float)dx[u] * fm);
+                kahan_add(sya, cya, (float)dy[u] * fm);
+                kahan_add(sza, cza, (float)dz[u] * fm);
+            }
+        }
+    }
+    if (PREC != PREC_FP32_KAHAN) return make_double4(fx, fy, fz, 0.0);
+    return make_double4((double)sxa + (double)cxa, (double)sya + (double)cya, (double)sza + (double)cza, 0.0);
+}
+
+template <int PREC, int SHIFT, int UNIT>
 __global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force) {
     extern __shared__ double dyn[];
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -90,78 +202,15 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    double* sx = dyn;
-    double* sy = dyn + a.wcap;
-    double* sz = dyn + 2 * a.wcap;
-    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, nullptr);
+    const bool staged = stage_tile_aos(W, a.wcap, spos, dyn);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
-    const int w = w0 + at;
-    const int i = work[w];
-    const double4 pi = pos[i];
-    const int cnt = nbc[w];
-    const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
-    const int ng = (cnt + 7) >> 3;
-    for (int gi = 0; gi < ng; ++gi) {
-        const Nb8 e = unpack8(lp[(size_t)gi * 64]);
-        const int k0 = gi * 8;
-        double xs[8], ys[8], zs[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int t = (k0 + u < cnt) ? e.t[u] : 0;
-            if (staged) {
-                xs[u] = sx[t];
-                ys[u] = sy[t];
-                zs[u] = sz[t];
-            } else {
-                const double4 p = spos[win_index(t, W)];
-                xs[u] = p.x;
-                ys[u] = p.y;
-                zs[u] = p.z;
-            }
-        }
-        // branch-free: all 8 pair terms are evaluated (independent chains the
-        // scheduler can interleave), then accumulated in list order through
-        // selects, so the FP64 sum is the sequential one
-        bool ok[8];
-        double ddx[8], ddy[8], ddz[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            ddx[u] = pi.x - xs[u];
-            ddy[u] = pi.y - ys[u];
-            ddz[u] = pi.z - zs[u];
-            const double rsq = ddx[u] * ddx[u] + ddy[u] * ddy[u] + ddz[u] * ddz[u];
-            ok[u] = (k0 + u < cnt) && (rsq < a.rc2);
-            xs[u] = ok[u] ? rsq : 1.0;  // reuse as the (safe) rsq
-        }
-        if (PREC == PREC_FP64) {
-            double ff[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) ff[u] = lj_f64<SHIFT>(a, xs[u]);
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double nx = fx + ddx[u] * ff[u];
-                const double ny = fy + ddy[u] * ff[u];
-                const double nz = fz + ddz[u] * ff[u];
-                fx = ok[u] ? nx : fx;
-                fy = ok[u] ? ny : fy;
-                fz = ok[u] ? nz : fz;
-            }
-        } else {
-            float ff[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) ff[u] = ok[u] ? lj_f32<SHIFT>(a, (float)xs[u]) : 0.0f;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                two_sum_acc(sxa, cxa, (float)ddx[u] * ff[u]);
-                two_sum_acc(sya, cya, (float)ddy[u] * ff[u]);
-                two_sum_acc(sza, cza, (float)ddz[u] * ff[u]);
-            }
-        }
-    }
-    if (PREC == PREC_FP64) force[i] = make_double4(fx, fy, fz, 0.0);
-    else force[i] = make_double4((double)sxa + (double)cxa, (double)sya + (double)cya, (double)sza + (double)cza, 0.0);
+        const int w = w0 + at;
+        const int i = work[w];
+        const double4 pi = pos[i];
+        const int cnt = nbc[w];
+        const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+        force[i] = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, dyn, spos, W, a)
+                          : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, dyn, spos, W, a);
     }
 }
 
@@ -495,13 +544,21 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma) {
     s.timer.begin(K_FORCE, s.stream, &ev);
     if (!half) {
         double4* F = s.force.as<double4>();
+        const size_t lds = lds_bytes(s, false);
+        const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
+#define TMD_LJ_FULL(P, SH, U) \
+    hipLaunchKernelGGL((k_lj_full<P, SH, U>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, F)
         if (s.precision == PREC_FP64) {
-            if (shift) hipLaunchKernelGGL((k_lj_full<PREC_FP64, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
-            else hipLaunchKernelGGL((k_lj_full<PREC_FP64, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
+            if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }
+            else { if (unit) TMD_LJ_FULL(PREC_FP64, 0, 1); else TMD_LJ_FULL(PREC_FP64, 0, 0); }
+        } else if (s.precision == PREC_FP32_MIXED) {
+            if (shift) { if (unit) TMD_LJ_FULL(PREC_FP32_MIXED, 1, 1); else TMD_LJ_FULL(PREC_FP32_MIXED, 1, 0); }
+            else { if (unit) TMD_LJ_FULL(PREC_FP32_MIXED, 0, 1); else TMD_LJ_FULL(PREC_FP32_MIXED, 0, 0); }
         } else {
-            if (shift) hipLaunchKernelGGL((k_lj_full<PREC_FP32_MIXED, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
-            else hipLaunchKernelGGL((k_lj_full<PREC_FP32_MIXED, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, false), s.stream, TMD_LIST_ARGS, a, F);
+            if (shift) { if (unit) TMD_LJ_FULL(PREC_FP32_KAHAN, 1, 1); else TMD_LJ_FULL(PREC_FP32_KAHAN, 1, 0); }
+            else { if (unit) TMD_LJ_FULL(PREC_FP32_KAHAN, 0, 1); else TMD_LJ_FULL(PREC_FP32_KAHAN, 0, 0); }
         }
+#undef TMD_LJ_FULL
     } else {
         ensure_acc(s);
         int* flag = s.counters.as<int>() + 4;

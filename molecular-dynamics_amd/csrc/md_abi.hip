@@ -594,7 +594,7 @@ int tmd_device_info(char* buf, int len) {
     return prop.multiProcessorCount;
 }
 
-void tmd_set_precision(int mode) { sim().precision = mode ? PREC_FP32_MIXED : PREC_FP64; }
+void tmd_set_precision(int mode) { sim().precision = mode == 2 ? PREC_FP32_KAHAN : (mode ? PREC_FP32_MIXED : PREC_FP64); }
 void tmd_set_half_mode(int mode) { sim().half_mode = mode ? HALF_REF_LITERAL : HALF_GHOST_FULL; }
 void tmd_set_ghost_mode(int mode) { sim().ghost_mode = mode ? GHOST_REF_LAG : GHOST_FRESH; }
 void tmd_set_sync(int on) { sim().sync_calls = on ? 1 : 0; }

@@ -36,7 +36,7 @@ bool has_error();
 
 // ---------------------------------------------------------------------------
 // modes
-enum Precision { PREC_FP64 = 0, PREC_FP32_MIXED = 1 };
+enum Precision { PREC_FP64 = 0, PREC_FP32_MIXED = 1, PREC_FP32_KAHAN = 2 };
 enum HalfMode { HALF_GHOST_FULL = 0, HALF_REF_LITERAL = 1 };
 enum GhostMode { GHOST_FRESH = 0, GHOST_REF_LAG = 1 };
 

@@ -129,4 +129,21 @@ __device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const dou
     return staged;
 }
 
+// Stage the window's FP64 positions interleaved (x, y, z per atom) into LDS
+// if it fits wcap atoms.  Whole block.  Returns whether it was staged.
+__device__ __forceinline__ bool stage_tile_aos(const TileWin& W, int wcap, const double4* __restrict__ spos, double* sp) {
+    const int S = win_size(W);
+    const bool staged = S <= wcap;
+    if (staged) {
+        for (int t = threadIdx.x; t < S; t += blockDim.x) {
+            const double4 p = spos[win_index(t, W)];
+            sp[3 * t + 0] = p.x;
+            sp[3 * t + 1] = p.y;
+            sp[3 * t + 2] = p.z;
+        }
+    }
+    __syncthreads();
+    return staged;
+}
+
 }  // namespace tmd

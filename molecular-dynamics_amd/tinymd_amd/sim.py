@@ -9,7 +9,7 @@ import numpy as np
 
 from ._lib import Vector3D, check, load
 
-PRECISION = {"fp64": 0, "fp32mixed": 1, "fp32": 1}
+PRECISION = {"fp64": 0, "fp32mixed": 1, "fp32": 1, "fp32kahan": 2}
 HALF_MODE = {"ghost_full": 0, "ref_literal": 1}
 GHOST_MODE = {"fresh": 0, "ref_lag": 1}
 KERNELS = {"force": 0, "integrate1": 1, "integrate2": 2, "ghost_sync": 3, "neighbor": 4, "bin": 5,

@@ -124,11 +124,11 @@ def test_lj_half_fp64(tm, half_mode):
         assert rms < 1e-12 and mx < 1e-11, (rms, mx)
 
 
-@pytest.mark.parametrize("half", [False, True])
-def test_lj_fp32_mixed_error(tm, half):
-    announce(f"LJ FP32-mixed error half={half}")
+@pytest.mark.parametrize("half,precision", [(False, "fp32mixed"), (True, "fp32mixed"), (False, "fp32kahan")])
+def test_lj_fp32_mixed_error(tm, half, precision):
+    announce(f"LJ {precision} error half={half}")
     aabb, m, x, v = perturbed_fcc(10, amp=0.15, seed=6)
-    sim, orc = make_pair(aabb, m, x, v, precision="fp32mixed")
+    sim, orc = make_pair(aabb, m, x, v, precision=precision)
     setup_both(sim, orc, half)
     sim.lennard_jones(half, RC)
     orc.lj(half, RC)
