@@ -92,6 +92,28 @@ int tmd_generate_fcc(int nx, int ny, int nz, double potential_minimum, int half,
 int tmd_generate_granular_gas(int nc, double potential_minimum, unsigned long long seed, double vmax, double* mass,
                               struct Vector3D* pos, struct Vector3D* vel, int cap);
 
+/* --- differentiable trajectories (no counterpart in the reference) ---------
+ * Loss of an LJ velocity-Verlet trajectory of `steps` steps from the current
+ * state (x0, v0), reneighbouring every `reneigh_every` steps with full lists
+ * as tmd_run_lj does:
+ *   Ubar = (1/T) sum_{t=1..T} PE_t / N,
+ *   Pbar = (1/T) sum_{t=1..T} (sum_i v_i^2 + W_t) / (3 V),  W = sum_pairs r.F,
+ *   L    = (Ubar - u_target)^2 + (Pbar - p_target)^2   (unit masses).
+ * The current precision and LJ shift modes apply; with the smooth cutoff
+ * (tmd_set_lj_shift(1)) the trajectory is differentiable and the adjoint is
+ * its exact gradient.  Both calls leave the state, ghosts, list and forces
+ * as they were at x0 (full list).  Return 1 on success, 0 on error. */
+int tmd_traj_loss(int steps, int reneigh_every, double dt, double cutoff, double skin, double epsilon, double sigma,
+                  double* out2 /* {Ubar, Pbar} */);
+/* Reverse-mode gradient of L: state checkpoints every `reneigh_every` steps,
+ * each interval recomputed forward and reversed with the hand-written
+ * Hessian-vector-product kernel.  out3 = {L, Ubar, Pbar}; grad_theta2 =
+ * {dL/depsilon, dL/dsigma}; grad_x0 / grad_v0 (n entries, may be NULL) =
+ * dL/dx0, dL/dv0. */
+int tmd_traj_adjoint(int steps, int reneigh_every, double dt, double cutoff, double skin, double epsilon, double sigma,
+                     double u_target, double p_target, double* out3, double* grad_theta2, struct Vector3D* grad_x0,
+                     struct Vector3D* grad_v0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,15 +27,11 @@
 // accumulates in FP32 with a compensation term instead.
 #include "tmd_internal.hpp"
 #include "tmd_window.hpp"
+#include "tmd_lj.hpp"
 
 namespace tmd {
 
-struct LJArgs {
-    double rc2, rc, eps, sigma6, frc, urc;
-    float epsf, sigma6f, frcf;
-    int cap, nwork, n, wcap;
-    TileGeom g;
-};
+
 
 // error-free transformation: s + x = t + e exactly (Knuth TwoSum)
 __device__ __forceinline__ void two_sum_acc(float& s, float& c, float x) {
@@ -52,7 +48,7 @@ __device__ __forceinline__ double lj_f64(const LJArgs& a, double rsq) {
     const double sr2 = 1.0 / rsq;
     const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
     double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
-    if (SHIFT) f = f - a.frc / sqrt(rsq);
+    if (SHIFT) f = f - a.fc - (rsq - a.rc2) * a.dfc;
     return f;
 }
 
@@ -61,7 +57,7 @@ __device__ __forceinline__ float lj_f32(const LJArgs& a, float rsq) {
     const float sr2 = __builtin_amdgcn_rcpf(rsq);
     const float sr6 = sr2 * sr2 * sr2 * a.sigma6f;
     float f = 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
-    if (SHIFT) f = f - a.frcf * __builtin_amdgcn_rsqf(rsq);
+    if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
     return f;
 }
 
@@ -91,7 +87,7 @@ __device__ __forceinline__ double lj_f64u(const LJArgs& a, double rsq) {
     const double sr2 = 1.0 / rsq;
     const double sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6;
     double f = UNIT ? 48.0 * sr6 * (sr6 - 0.5) * sr2 : 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
-    if (SHIFT) f = f - a.frc / sqrt(rsq);
+    if (SHIFT) f = f - a.fc - (rsq - a.rc2) * a.dfc;
     return f;
 }
 
@@ -100,7 +96,7 @@ __device__ __forceinline__ float lj_f32u(const LJArgs& a, float rsq) {
     const float sr2 = __builtin_amdgcn_rcpf(rsq);
     const float sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6f;
     float f = UNIT ? 48.0f * sr6 * (sr6 - 0.5f) * sr2 : 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
-    if (SHIFT) f = f - a.frcf * __builtin_amdgcn_rsqf(rsq);
+    if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
     return f;
 }
 
@@ -344,9 +340,9 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARA
                 double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
                 double u = 4.0 * a.eps * sr6 * (sr6 - 1.0);
                 if (SHIFT) {
-                    const double r = sqrt(rsq);
-                    f = f - a.frc / r;
-                    u = u - a.urc + (r - a.rc) * a.frc;
+                    const double ds = rsq - a.rc2;
+                    f = f - a.fc - ds * a.dfc;
+                    u = u - a.uc + ds * (0.5 * a.fc) + ds * ds * (0.25 * a.dfc);
                 }
                 const double wt = half ? (j < a.n ? 1.0 : 0.5) : 0.5;
                 pe += wt * u;
@@ -482,27 +478,6 @@ __global__ void __launch_bounds__(kTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, co
 }
 
 // ---------------------------------------------------------------------------
-static LJArgs make_lj_args(const Sim& s, double rc, double eps, double sigma) {
-    LJArgs a;
-    a.rc = rc;
-    a.rc2 = rc * rc;
-    a.eps = eps;
-    const double s3 = sigma * (sigma * sigma);  // pow(sigma, 6): utils/utilities.impala:27-36
-    a.sigma6 = s3 * s3;
-    const double sr2 = 1.0 / a.rc2;
-    const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
-    a.frc = 48.0 * sr6 * (sr6 - 0.5) * sr2 * eps * rc;
-    a.urc = 4.0 * eps * sr6 * (sr6 - 1.0);
-    a.epsf = (float)eps;
-    a.sigma6f = (float)a.sigma6;
-    a.frcf = (float)a.frc;
-    a.cap = s.nb_cap;
-    a.nwork = s.n;
-    a.n = s.n;
-    a.wcap = window_cap(s, true);
-    a.g = s.tiles;
-    return a;
-}
 
 static void ensure_acc(Sim& s) {
     const size_t need = sizeof(unsigned long long) * 3 * (size_t)(s.n > 0 ? s.n : 1);
@@ -611,19 +586,25 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     if (half && s.sync_calls) check_fix_flag(s);
 }
 
-void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs) {
-    *pe = 0.0; *vir = 0.0; *pairs = 0;
-    if (has_error() || !s.nb_valid || s.n == 0) return;
+void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part) {
     LJArgs a = make_lj_args(s, rc, eps, sigma);
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     refresh_mirror(s);
-    s.red.reserve(sizeof(double) * (3 * (size_t)nb + 8), false, s.stream);
-    double* part = s.red.as<double>();
-    double* out = part + 3 * (size_t)nb;
     if (s.lj_shift)
         hipLaunchKernelGGL(k_lj_energy<1>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
     else
         hipLaunchKernelGGL(k_lj_energy<0>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
+    TMD_HIP(hipGetLastError());
+}
+
+void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs) {
+    *pe = 0.0; *vir = 0.0; *pairs = 0;
+    if (has_error() || !s.nb_valid || s.n == 0) return;
+    const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    s.red.reserve(sizeof(double) * (3 * (size_t)nb + 8), false, s.stream);
+    double* part = s.red.as<double>();
+    double* out = part + 3 * (size_t)nb;
+    launch_lj_energy(s, rc, eps, sigma, part);
     reduce_partials(s, part, nb, 3, out);
     double host[3] = {0, 0, 0};
     TMD_HIP(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, s.stream));

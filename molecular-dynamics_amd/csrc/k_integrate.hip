@@ -97,13 +97,20 @@ void enforce_pbc(Sim& s) {
     TMD_HIP(hipGetLastError());
 }
 
+int launch_kinetic(Sim& s, double* part) {
+    const int nt = 256, nb = div_up(s.n, nt);
+    hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(nt), 0, s.stream, (const double4*)s.vel.as<double4>(), s.n, part);
+    TMD_HIP(hipGetLastError());
+    return nb;
+}
+
 double kinetic_sum(Sim& s) {
     if (has_error() || s.n == 0) return 0.0;
-    const int nt = 256, nb = div_up(s.n, nt);
+    const int nb = div_up(s.n, 256);
     s.red.reserve(sizeof(double) * ((size_t)nb + 8), false, s.stream);
     double* part = s.red.as<double>();
     double* out = part + nb;
-    hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(nt), 0, s.stream, (const double4*)s.vel.as<double4>(), s.n, part);
+    launch_kinetic(s, part);
     reduce_partials(s, part, nb, 1, out);
     double host = 0.0;
     TMD_HIP(hipMemcpyAsync(&host, out, sizeof host, hipMemcpyDeviceToHost, s.stream));

@@ -421,7 +421,9 @@ void md_deallocate_grid() {
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
                       &s.cell_count, &s.cell_start, &s.cell_parts, &s.work_start, &s.work, &s.tile_start, &s.spos,
                       &s.nb_count, &s.nb_list,
-                      &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf};
+                      &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
+                      &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
+                      &s.adj_acc};
     for (DevBuf* b : bufs) b->release();
     s.n = s.ng = 0;
     s.cap_all = 0;
@@ -770,6 +772,33 @@ int tmd_run_lj(int steps, int reneigh_every, double dt, bool half, double cutoff
 int tmd_run_dem(int steps, int reneigh_every, double dt, bool half, double cutoff, double skin, double damping_n, double damping_t,
                 double stiffness, double friction) {
     return run_loop(steps, reneigh_every, dt, half, cutoff, skin, 1, damping_n, damping_t, stiffness, friction);
+}
+
+int tmd_traj_loss(int steps, int reneigh_every, double dt, double cutoff, double skin, double epsilon, double sigma,
+                  double* out2) {
+    Sim& s = sim();
+    out2[0] = out2[1] = 0.0;
+    if (!ready(s)) return 0;
+    const int saved_sync = s.sync_calls;
+    s.sync_calls = 0;
+    const bool ok = traj_loss(s, steps, reneigh_every, dt, cutoff, skin, epsilon, sigma, &out2[0], &out2[1]);
+    s.sync_calls = saved_sync;
+    return ok ? 1 : 0;
+}
+
+int tmd_traj_adjoint(int steps, int reneigh_every, double dt, double cutoff, double skin, double epsilon, double sigma,
+                     double u_target, double p_target, double* out3, double* grad_theta2, struct Vector3D* grad_x0,
+                     struct Vector3D* grad_v0) {
+    Sim& s = sim();
+    out3[0] = out3[1] = out3[2] = 0.0;
+    grad_theta2[0] = grad_theta2[1] = 0.0;
+    if (!ready(s)) return 0;
+    const int saved_sync = s.sync_calls;
+    s.sync_calls = 0;
+    const bool ok = traj_adjoint(s, steps, reneigh_every, dt, cutoff, skin, epsilon, sigma, u_target, p_target, out3,
+                                 grad_theta2, reinterpret_cast<double*>(grad_x0), reinterpret_cast<double*>(grad_v0));
+    s.sync_calls = saved_sync;
+    return ok ? 1 : 0;
 }
 
 void tmd_profile_enable(int on) {

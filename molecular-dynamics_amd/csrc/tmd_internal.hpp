@@ -121,6 +121,10 @@ struct Sim {
     DevBuf nb_count, nb_list;
     DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
+    // trajectory loss and adjoint: initial state, checkpoints, interval
+    // states, adjoints (double4 per local atom), partial sums, accumulators
+    DevBuf adj_x0, adj_v0, adj_ckx, adj_ckv, adj_segx, adj_segv;
+    DevBuf adj_xbar, adj_vbar, adj_fbar, adj_part, adj_acc;
 
     hipStream_t stream = nullptr;
     KernelTimer timer;
@@ -162,11 +166,22 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
 void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs);
 
 void reduce_partials(Sim& s, const double* partials, int nblocks, int width, double* out);
+// per-tile (PE, W, pairs) partials of the current full/half list into part[3 * ntiles]
+void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part);
 
 // integration
 void initial_integration(Sim& s, double dt);
 void final_integration(Sim& s, double dt);
 void enforce_pbc(Sim& s);
 double kinetic_sum(Sim& s);  // sum m v.v over locals (deterministic)
+int launch_kinetic(Sim& s, double* part);  // per-block partials of sum v.v; returns the block count
+
+// trajectory loss (Ubar, Pbar) and its adjoint (k_adjoint.hip)
+bool traj_loss(Sim& s, int steps, int ck, double dt, double rc, double skin, double eps, double sigma, double* ubar,
+               double* pbar);
+// loss3 = {L, Ubar, Pbar}; grad_theta = {dL/deps, dL/dsigma}; grad_x, grad_v
+// (3 n doubles each, may be null) = dL/dx0, dL/dv0
+bool traj_adjoint(Sim& s, int steps, int ck, double dt, double rc, double skin, double eps, double sigma, double u_target,
+                  double p_target, double* loss3, double* grad_theta, double* grad_x, double* grad_v);
 
 }  // namespace tmd

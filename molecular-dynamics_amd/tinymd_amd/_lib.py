@@ -91,6 +91,8 @@ SIGNATURES = {
     "tmd_profile_reset": (None, []),
     "tmd_generate_fcc": (_i, [_i, _i, _i, _d, _i, _dp, _vp, _vp, _i]),
     "tmd_generate_granular_gas": (_i, [_i, _d, ctypes.c_ulonglong, _d, _dp, _vp, _vp, _i]),
+    "tmd_traj_loss": (_i, [_i, _i, _d, _d, _d, _d, _d, _dp]),
+    "tmd_traj_adjoint": (_i, [_i, _i, _d, _d, _d, _d, _d, _d, _d, _dp, _dp, _vp, _vp]),
 }
 
 _lib = None

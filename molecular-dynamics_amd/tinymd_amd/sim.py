@@ -180,6 +180,30 @@ class Simulation:
         check()
         return done
 
+    # ---- differentiable trajectories
+    def traj_loss(self, steps, reneigh=20, dt=0.005, cutoff=2.5, skin=0.3, eps=1.0, sigma=1.0):
+        """(Ubar, Pbar) of an LJ trajectory from the current state; the state is left unchanged."""
+        out = np.zeros(2)
+        self.L.tmd_traj_loss(steps, reneigh, dt, cutoff, skin, eps, sigma, _dptr(out))
+        check()
+        return float(out[0]), float(out[1])
+
+    def traj_adjoint(self, steps, reneigh=20, dt=0.005, cutoff=2.5, skin=0.3, eps=1.0, sigma=1.0,
+                     u_target=0.0, p_target=0.0, grad_state=True):
+        """Loss L = (Ubar - u_target)^2 + (Pbar - p_target)^2 and its gradient.
+
+        Returns dict(loss, ubar, pbar, grad_eps, grad_sigma, grad_x0, grad_v0)."""
+        out = np.zeros(3)
+        gt = np.zeros(2)
+        n = self.nlocal
+        gx = np.zeros((n, 3)) if grad_state else None
+        gv = np.zeros((n, 3)) if grad_state else None
+        self.L.tmd_traj_adjoint(steps, reneigh, dt, cutoff, skin, eps, sigma, u_target, p_target, _dptr(out),
+                                _dptr(gt), None if gx is None else _vptr(gx), None if gv is None else _vptr(gv))
+        check()
+        return dict(loss=float(out[0]), ubar=float(out[1]), pbar=float(out[2]), grad_eps=float(gt[0]),
+                    grad_sigma=float(gt[1]), grad_x0=gx, grad_v0=gv)
+
     def barrier(self):
         self.L.md_barrier(); check()
 

@@ -71,6 +71,8 @@ def lib():
             "orc_run_lj": (_i, [_vp, _i, _i, _d, _i, _i, _d, _d, _d, _d, _i]),
             "orc_run_dem": (_i, [_vp, _i, _i, _d, _i, _i, _d, _d, _d, _d, _d, _d, _d]),
             "orc_setup": (None, [_vp, _i, _i, _d]),
+            "orc_traj_thermo": (None, [_vp, _i, _i, _d, _d, _d, _d, _d, _i, _dp]),
+            "orc_traj_adjoint": (None, [_vp, _i, _i, _d, _d, _d, _d, _d, _i, _d, _d, _dp, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -232,6 +234,23 @@ class OracleSim:
     def run_lj(self, steps, reneigh=20, dt=0.005, half=False, half_mode=HALF_GHOST_FULL,
                rc=2.5, skin=0.3, eps=1.0, sigma=1.0, shift=0):
         return self.L.orc_run_lj(self.h, steps, reneigh, dt, int(half), half_mode, rc, skin, eps, sigma, shift)
+
+    def traj_thermo(self, steps, reneigh=20, dt=0.005, rc=2.5, skin=0.3, eps=1.0, sigma=1.0, shift=1):
+        """(Ubar, Pbar): time averages of PE/N and P over `steps` steps."""
+        out = np.zeros(2)
+        self.L.orc_traj_thermo(self.h, steps, reneigh, dt, rc, skin, eps, sigma, shift, _ptr(out))
+        return float(out[0]), float(out[1])
+
+    def traj_adjoint(self, steps, reneigh=20, dt=0.005, rc=2.5, skin=0.3, eps=1.0, sigma=1.0, shift=1,
+                     u_target=0.0, p_target=0.0):
+        """Loss (Ubar - u_target)^2 + (Pbar - p_target)^2 and its gradient; the
+        state is left at x0."""
+        n = self.n
+        out = np.zeros(3); gt = np.zeros(2); gx = np.zeros((n, 3)); gv = np.zeros((n, 3))
+        self.L.orc_traj_adjoint(self.h, steps, reneigh, dt, rc, skin, eps, sigma, shift, u_target, p_target,
+                                _ptr(out), _ptr(gt), _ptr(gx), _ptr(gv))
+        return dict(loss=float(out[0]), ubar=float(out[1]), pbar=float(out[2]), grad_eps=float(gt[0]),
+                    grad_sigma=float(gt[1]), grad_x0=gx, grad_v0=gv)
 
     def run_dem(self, steps, reneigh=20, dt=0.001, half=True, half_mode=HALF_GHOST_FULL,
                 rc=1.0, skin=0.3, radius=0.5, gn=0.5, gt=0.2, k=100.0, mu=0.3):

@@ -27,8 +27,10 @@
 //   DEM                            -> closed-form restitution of a head-on pair
 //   velocity Verlet                -> free flight / harmonic closed forms,
 //                                     time reversibility
-//   adjoint                        -> central finite differences of the
-//                                     oracle's own loss (FP64)
+//   trajectory loss                -> orc_run_lj trajectory + orc_thermo per step
+//   adjoint                        -> central finite differences (h = 1e-6)
+//                                     of the oracle's own loss, FP64, for
+//                                     eps, sigma, x0 and v0 directions
 
 #include <algorithm>
 #include <cmath>
@@ -360,10 +362,14 @@ void neighbors(Sim* s, int half, int half_mode, double cutoff) {
 // ---------------------------------------------------------------------------
 // Pair kernels.  The LJ closure is core/runtime.impala:188-201 (force only);
 // energy U = 4 eps (sr12 - sr6) and the pair virial r.F = f * rsq are this
-// project's additions (the reference computes neither).  shift=1 selects the
-// shifted-force form F_sf(r) = F(r) - F(rc) (force and energy continuous at rc).
+// project's additions (the reference computes neither).  In terms of
+// s = r^2 the force is f(s) * del with f = 48 eps sr6 (sr6 - 1/2) / s.
+// shift=1 selects the smoothly truncated form used for differentiable runs:
+//   f_s(s) = f(s) - f(sc) - (s - sc) f'(sc),      sc = rc^2
+//   u_s(s) = u(s) - u(sc) + (s - sc) f(sc)/2 + (s - sc)^2 f'(sc)/4
+// so that the force and its derivative vanish at rc (u_s' = -f_s / 2).
 struct LJ {
-    double rc2, rc, eps, sigma6, frc, urc;  // frc = |F(rc)|, urc = U(rc)
+    double rc2, rc, eps, sigma6, fc, dfc, uc;
     int shift;
 };
 
@@ -374,8 +380,9 @@ inline LJ make_lj(double rc, double eps, double sigma, int shift) {
     p.sigma6 = s3 * s3;
     double sr2 = 1.0 / p.rc2;
     double sr6 = sr2 * sr2 * sr2 * p.sigma6;
-    p.frc = 48.0 * sr6 * (sr6 - 0.5) * sr2 * eps * rc;
-    p.urc = 4.0 * eps * sr6 * (sr6 - 1.0);
+    p.fc = 48.0 * sr6 * (sr6 - 0.5) * sr2 * eps;
+    p.dfc = 24.0 * eps * sr2 * sr2 * sr6 * (4.0 - 14.0 * sr6);
+    p.uc = 4.0 * eps * sr6 * (sr6 - 1.0);
     return p;
 }
 
@@ -386,9 +393,9 @@ inline double lj_pair(const LJ& p, double rsq, double* u, double* w) {
     double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * p.eps;
     double e = 4.0 * p.eps * sr6 * (sr6 - 1.0);
     if (p.shift) {
-        double r = std::sqrt(rsq);
-        f = f - p.frc / r;
-        e = e - p.urc + (r - p.rc) * p.frc;
+        const double ds = rsq - p.rc2;
+        f = f - p.fc - ds * p.dfc;
+        e = e - p.uc + ds * (0.5 * p.fc) + ds * ds * (0.25 * p.dfc);
     }
     *u = e;
     *w = f * rsq;
@@ -718,6 +725,144 @@ int orc_run_dem(void* h, int steps, int reneigh_every, double dt, int half, int 
         final_integration(s, dt);
     }
     return rebuilds;
+}
+
+// Time-averaged thermo over a velocity-Verlet trajectory (LJ): the loss of
+// the differentiable runs.  After each of `steps` steps (main.cpp step order,
+// rebuild every reneigh_every) accumulate U_t = PE_t / N and
+// P_t = (sum v^2 + W_t) / (3V).  out = {Ubar, Pbar}.
+void orc_traj_thermo(void* h, int steps, int reneigh_every, double dt, double rc, double skin, double eps, double sigma,
+                     int shift, double* out) {
+    Sim* s = (Sim*)h;
+    LJ p = make_lj(rc, eps, sigma, shift);
+    const double vol = s->len[0] * s->len[1] * s->len[2];
+    double su = 0.0, sp = 0.0;
+    for (int j = 0; j < steps; ++j) {
+        initial_integration(s, dt);
+        if ((j + 1) % reneigh_every == 0) rebuild(s, 0, HALF_GHOST_FULL, rc + skin);
+        else sync_ghosts(s);
+        compute_lj(s, 0, p);
+        final_integration(s, dt);
+        double t = temperature(s);
+        double mv2 = t * (double)(3 * s->n - 3);
+        su += s->pe / (double)s->n;
+        sp += (mv2 + s->virial) / (3.0 * vol);
+    }
+    out[0] = su / (double)steps;
+    out[1] = sp / (double)steps;
+}
+
+// Reverse-mode gradient of the orc_traj_thermo loss
+//   L = (Ubar - u_t)^2 + (Pbar - p_t)^2
+// with respect to eps, sigma, x0 and v0.  Every step's state (positions with
+// ghosts, velocities, ghost table, list) is kept; the steps are undone in
+// reverse order.  J_F^T is applied in scatter form, one list entry (i, j) at
+// a time: with M = f(s) I + 2 f'(s) r r^T, xbar_i += M Fbar_i and
+// xbar_owner(j) -= M Fbar_i (F_i = sum_j f(s_ij) r_ij); nothing assumes that
+// the list is symmetric.  The loss terms per entry are the derivatives of
+// PE/2 = u(s)/2 and W/2 = s f(s)/2, and dL/dv_t = 2 cW v_t.
+void orc_traj_adjoint(void* h, int steps, int reneigh_every, double dt, double rc, double skin, double eps,
+                      double sigma, int shift, double u_target, double p_target, double* out3, double* gtheta,
+                      double* gx, double* gv) {
+    Sim* s = (Sim*)h;
+    const Sim s0 = *s;
+    const LJ p = make_lj(rc, eps, sigma, shift);
+    const int n = s->n;
+    const double vol = s->len[0] * s->len[1] * s->len[2];
+    std::vector<Sim> traj;
+    traj.reserve((size_t)steps + 1);
+    traj.push_back(*s);
+    double su = 0.0, sp = 0.0;
+    for (int j = 0; j < steps; ++j) {
+        initial_integration(s, dt);
+        if ((j + 1) % reneigh_every == 0) rebuild(s, 0, HALF_GHOST_FULL, rc + skin);
+        else sync_ghosts(s);
+        compute_lj(s, 0, p);
+        final_integration(s, dt);
+        double mv2 = 0.0;
+        for (int i = 0; i < n; ++i) mv2 += s->v[3 * i] * s->v[3 * i] + s->v[3 * i + 1] * s->v[3 * i + 1] + s->v[3 * i + 2] * s->v[3 * i + 2];
+        su += s->pe / (double)n;
+        sp += (mv2 + s->virial) / (3.0 * vol);
+        traj.push_back(*s);
+    }
+    const double ubar = su / steps, pbar = sp / steps;
+    out3[0] = (ubar - u_target) * (ubar - u_target) + (pbar - p_target) * (pbar - p_target);
+    out3[1] = ubar;
+    out3[2] = pbar;
+    const double cU = 2.0 * (ubar - u_target) / ((double)steps * n);
+    const double cW = 2.0 * (pbar - p_target) / (3.0 * vol * steps);
+    // d/dsigma of f(sc), f'(sc), u(sc): f, f', u are sums of sigma^6 and
+    // sigma^12 terms, so d/dsigma = (6 A6 + 12 A12) / sigma per term
+    const double sc = rc * rc, isc = 1.0 / sc;
+    const double s6c = p.sigma6 * isc * isc * isc, s12c = s6c * s6c;
+    const double dfc_ds = 48.0 * eps * isc * (12.0 * s12c - 3.0 * s6c) / sigma;        // f = 48e(s12 - s6/2)/s
+    const double ddfc_ds = 48.0 * eps * isc * isc * (-84.0 * s12c + 12.0 * s6c) / sigma;  // f' = 48e(-7 s12 + 2 s6)/s^2
+    const double duc_ds = 4.0 * eps * (12.0 * s12c - 6.0 * s6c) / sigma;                 // u = 4e(s12 - s6)
+    std::vector<double> xb(3 * (size_t)n, 0.0), vb(3 * (size_t)n, 0.0), fb(3 * (size_t)n, 0.0);
+    double ge = 0.0, gs = 0.0;
+    const double hdt = 0.5 * dt;
+    for (int t = steps; t >= 0; --t) {
+        const Sim& S = traj[(size_t)t];
+        const bool loss = t >= 1;
+        if (loss) {
+            for (int k = 0; k < 3 * n; ++k) {
+                vb[k] += 2.0 * cW * S.v[k];
+                fb[k] += hdt * vb[k];
+            }
+        }
+        const int cap = S.nb_cap;
+        for (int i = 0; i < n; ++i) {
+            const double* xi = &S.x[3 * i];
+            const double* Fi = &fb[3 * i];
+            for (int k = 0; k < S.nb_count[i]; ++k) {
+                const int j = S.nb_list[(size_t)i * cap + k];
+                const int o = j < n ? j : S.g_owner[j - n];
+                const double r[3] = {xi[0] - S.x[3 * j], xi[1] - S.x[3 * j + 1], xi[2] - S.x[3 * j + 2]};
+                const double q = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                if (!(q < p.rc2)) continue;
+                // pair terms written as powers of sigma^6 / q^3
+                const double iq = 1.0 / q, s6 = p.sigma6 * iq * iq * iq, s12 = s6 * s6;
+                double f = 48.0 * eps * (s12 - 0.5 * s6) * iq;
+                double df = 48.0 * eps * (-7.0 * s12 + 2.0 * s6) * iq * iq;
+                double u = 4.0 * eps * (s12 - s6);
+                double dfds = 48.0 * eps * (12.0 * s12 - 3.0 * s6) * iq / sigma;
+                double duds = 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sigma;
+                if (shift) {
+                    const double d = q - sc;
+                    f = f - p.fc - d * p.dfc;
+                    df = df - p.dfc;
+                    u = u - p.uc + 0.5 * d * p.fc + 0.25 * d * d * p.dfc;
+                    dfds = dfds - dfc_ds - d * ddfc_ds;
+                    duds = duds - duc_ds + 0.5 * d * dfc_ds + 0.25 * d * d * ddfc_ds;
+                }
+                const double rF = r[0] * Fi[0] + r[1] * Fi[1] + r[2] * Fi[2];
+                double cr = 2.0 * df * rF;
+                if (loss) cr += cU * (-0.5 * f) + cW * (f + q * df);
+                for (int e = 0; e < 3; ++e) {
+                    const double m = f * Fi[e] + cr * r[e];
+                    xb[3 * i + e] += m;
+                    xb[3 * o + e] -= m;
+                }
+                ge += (f / eps) * rF;
+                gs += dfds * rF;
+                if (loss) {
+                    ge += 0.5 * (cU * u + cW * q * f) / eps;
+                    gs += 0.5 * (cU * duds + cW * q * dfds);
+                }
+            }
+        }
+        if (loss) {
+            for (int k = 0; k < 3 * n; ++k) {
+                vb[k] += dt * xb[k];
+                fb[k] = hdt * vb[k];
+            }
+        }
+    }
+    gtheta[0] = ge;
+    gtheta[1] = gs;
+    if (gx) std::memcpy(gx, xb.data(), sizeof(double) * 3 * n);
+    if (gv) std::memcpy(gv, vb.data(), sizeof(double) * 3 * n);
+    *s = s0;
 }
 
 // Initial setup as main.cpp:691-700 (exchange, borders, distribute, lists).

@@ -37,6 +37,11 @@ int orc_run_lj(void* h, int steps, int reneigh_every, double dt, int half, int h
 int orc_run_dem(void* h, int steps, int reneigh_every, double dt, int half, int half_mode,
                 double rc, double skin, double radius, double gn, double gt, double k, double mu);
 void orc_setup(void* h, int half, int half_mode, double cutoff);
+void orc_traj_thermo(void* h, int steps, int reneigh_every, double dt, double rc, double skin, double eps, double sigma,
+                     int shift, double* out);
+void orc_traj_adjoint(void* h, int steps, int reneigh_every, double dt, double rc, double skin, double eps,
+                      double sigma, int shift, double u_target, double p_target, double* out3, double* gtheta,
+                      double* gx, double* gv);
 #ifdef __cplusplus
 }
 #endif

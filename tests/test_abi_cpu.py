@@ -1,0 +1,129 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+public headers declare, the headers compile as C and C++, the Python
+binding matches the headers, and the reference driver (main.cpp +
+initialize.h) links against the library unchanged (link only)."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+LIBDIR = os.path.join(ROOT, "molecular-dynamics_amd", "lib")
+LIB = os.path.join(LIBDIR, "libtinymd_amd.so")
+REF = "/root/reference"
+
+
+def header_functions():
+    names = []
+    for h in ("md_abi.h", "tmd_ext.h"):
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b((?:md|tmd)_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} not built")
+    return ctypes.CDLL(LIB)
+
+
+def test_header_declares_all_reference_entry_points():
+    names = header_functions()
+    md = [n for n in names if n.startswith("md_")]
+    # anydsl_includes.h:10-48 declares 39 md_* functions
+    assert len(md) == 39, md
+
+
+def test_library_exports_every_declared_symbol(lib):
+    out = subprocess.check_output(["nm", "-D", "--defined-only", LIB], text=True)
+    exported = set(re.findall(r" T (\w+)", out))
+    missing = [n for n in header_functions() if n not in exported]
+    assert not missing, missing
+    for n in header_functions():
+        getattr(lib, n)
+
+
+def test_python_binding_covers_headers():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+    from tinymd_amd._lib import SIGNATURES
+    assert set(header_functions()) == set(SIGNATURES), set(header_functions()) ^ set(SIGNATURES)
+
+
+@pytest.mark.parametrize("lang,compiler", [("c", "gcc"), ("c++", "g++")])
+def test_headers_compile(lang, compiler):
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t." + ("c" if lang == "c" else "cpp"))
+        with open(src, "w") as fh:
+            fh.write('#include "md_abi.h"\n#include "tmd_ext.h"\nint main(void) { return 0; }\n')
+        subprocess.check_call([compiler, "-std=c11" if lang == "c" else "-std=c++17", "-Wall", "-Werror",
+                               "-fsyntax-only", "-I" + INCLUDE, src])
+
+
+def test_headers_agree_with_reference_header():
+    """One translation unit including the reference's ABI header and ours:
+    any conflicting prototype is a compile error."""
+    if not os.path.isdir(REF):
+        pytest.skip("reference tree not present")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.cpp")
+        with open(src, "w") as fh:
+            fh.write('#include "anydsl_includes.h"\n#include "md_abi.h"\nint main() { return 0; }\n')
+        subprocess.check_call(["g++", "-std=c++17", "-fsyntax-only", "-I" + REF, "-I" + INCLUDE, src])
+
+
+def test_reference_driver_links_against_library():
+    """main.cpp and initialize.h from the reference tree, compiled and linked
+    against libtinymd_amd.so unchanged (not run: no GPU here)."""
+    if not os.path.isdir(REF):
+        pytest.skip("reference tree not present")
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "md")
+        subprocess.check_call(["g++", "-std=c++17", "-O0", "-I" + REF, os.path.join(REF, "main.cpp"), "-o", exe,
+                               "-L" + LIBDIR, "-ltinymd_amd", "-Wl,-rpath," + LIBDIR,
+                               "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,--allow-shlib-undefined"])
+        assert os.path.exists(exe)
+
+
+def test_no_gpu_fails_loudly(lib):
+    """Without a HIP device the library reports an error instead of falling
+    back to anything."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+    import numpy as np
+    import tinymd_amd as tm
+    name, cus = tm.device_info()
+    if cus > 0:
+        pytest.skip("a GPU is present")
+    m, x, v = tm.generate_fcc(4, 4, 4)
+    with pytest.raises(tm.TinyMDError):
+        tm.Simulation(m, x, v, tm.world_aabb(4, 4, 4), 2.8)
+    tm.load().tmd_clear_error()
+
+
+def test_host_lattice_generators_match_oracle():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+    import numpy as np
+    import pyoracle as po
+    import tinymd_amd as tm
+    for shape in ((4, 4, 4), (5, 3, 7), (8, 8, 8)):
+        m, x, v = tm.generate_fcc(*shape)
+        _, m2, x2, v2 = po.lattice(*shape)
+        assert np.array_equal(x, x2) and np.array_equal(v, v2) and np.array_equal(m, m2)
+    m, x, v = tm.generate_granular_gas(8, seed=3)
+    ijk = np.rint(x / (1.6796 / 2)).astype(int)
+    assert len(m) == (16 * 17 // 2) * 8  # i + j < 16 over a 16^3 vertex lattice, half the k parity
+    assert np.all(ijk[:, 0] + ijk[:, 1] < 16) and np.all(ijk.sum(axis=1) % 2 == 0)
+    assert np.all(np.abs(v) <= 1.0) and abs(v.mean()) < 0.05
+    m64, _, _ = tm.generate_granular_gas(64)
+    assert len(m64) == 528384
