@@ -102,6 +102,56 @@ def cpu_baseline(sample_steps=20):
                       f"{threads} OpenMP threads (list build and pair force threaded), {t1 - t0:.1f} s"}
 
 
+def adjoint_leg(tm, precision, nc=16, steps=200, ck=20, fd=False):
+    """Differentiable LJ: FCC nc^3 (16^3: 16,384 atoms), `steps` velocity-Verlet
+    steps with checkpoints every `ck`; loss (Ubar - U*)^2 + (Pbar - P*)^2 with
+    targets from the same initial state at eps = 1.05, sigma = 0.98.  Times the
+    forward loss (tmd_traj_loss) and the adjoint (tmd_traj_adjoint, which
+    includes its own forward pass and the interval recomputation)."""
+    sim = tm.Simulation.fcc(nc, precision=precision, lj_shift=True)
+    u_t, p_t = sim.traj_loss(steps, ck, DT, RC, SKIN, 1.05, 0.98)
+    sim.traj_loss(steps, ck, DT, RC, SKIN, 1.0, 1.0)  # warm-up
+    t0 = time.perf_counter()
+    ub, pb = sim.traj_loss(steps, ck, DT, RC, SKIN, 1.0, 1.0)
+    t1 = time.perf_counter()
+    r = sim.traj_adjoint(steps, ck, DT, RC, SKIN, 1.0, 1.0, u_t, p_t)
+    t2 = time.perf_counter()
+    r = sim.traj_adjoint(steps, ck, DT, RC, SKIN, 1.0, 1.0, u_t, p_t)
+    t3 = time.perf_counter()
+    out = {"atoms": sim.nlocal, "steps": steps, "checkpoint_every": ck, "forward_s": t1 - t0,
+           "adjoint_s": t3 - t2, "adjoint_over_forward": (t3 - t2) / (t1 - t0), "loss": r["loss"],
+           "grad_eps": r["grad_eps"], "grad_sigma": r["grad_sigma"],
+           "grad_x0_norm": float(np.linalg.norm(r["grad_x0"]))}
+    if fd:
+        # central differences (h = 1e-6) of the FP64 loss: the reference
+        # gradient for both precisions
+        h = 1e-6
+        sim.set_precision("fp64")
+
+        def loss(e, s):
+            u, p = sim.traj_loss(steps, ck, DT, RC, SKIN, e, s)
+            return (u - u_t) ** 2 + (p - p_t) ** 2
+        fe = (loss(1 + h, 1.0) - loss(1 - h, 1.0)) / (2 * h)
+        fs = (loss(1.0, 1 + h) - loss(1.0, 1 - h)) / (2 * h)
+        out["fd_grad_eps"], out["fd_grad_sigma"] = fe, fs
+        out["fd_rel_err_eps"] = abs(r["grad_eps"] - fe) / abs(fe)
+        out["fd_rel_err_sigma"] = abs(r["grad_sigma"] - fs) / abs(fs)
+    sim.close()
+    return out
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size (torch), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the force kernel."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "force_lj_fp32mixed_80cube_traffic.json")
     if os.path.exists(p):
@@ -117,7 +167,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--quick", action="store_true", help="headline variant only")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-trajectory leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU decomposition is design-only "
               "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
@@ -136,6 +192,7 @@ def main():
     print(f"# fp32mixed full: {r32}", file=sys.stderr, flush=True)
     nbytes, entries = force_kernel_bytes(sim32, "fp32mixed")
     err_rms, err_max = fp32_force_error(tm, sim32)
+    pairs_per_atom = sim32.thermo(RC)["pairs"] / 2.0 / sim32.nlocal  # unique in-cutoff pairs
     n = sim32.nlocal
     sim32.close()
     del sim32
@@ -146,6 +203,12 @@ def main():
             print(f"# {prec} half={half}: {r}", file=sys.stderr, flush=True)
             sim.close()
             del sim
+
+    adjoint = {}
+    if not args.no_adjoint:
+        for prec in ("fp64", "fp32mixed"):
+            adjoint[prec] = adjoint_leg(tm, prec, fd=not args.quick)
+            print(f"# adjoint {prec}: {adjoint[prec]}", file=sys.stderr, flush=True)
 
     value = r32["atom_steps_per_s"]
     t_force = r32["force_ms_avg"] * 1e-3
@@ -172,9 +235,12 @@ def main():
                      "kernel": "k_lj_full_f32", "bytes_per_launch": nbytes,
                      "avg_launch_ms": r32["force_ms_avg"]},
         "fp32_vs_fp64_force_err": {"rms_rel": err_rms, "max_abs_over_rms": err_max},
-        "pair_interactions_per_s": None,
+        "pair_interactions_per_s": value * pairs_per_atom,
+        "list_entries_per_s": value * entries / n,
         "variants": variants,
     }
+    if adjoint:
+        line["adjoint"] = adjoint
     if "fp64_full" in variants:
         line["fp32mixed_over_fp64"] = value / variants["fp64_full"]["atom_steps_per_s"]
     if not args.no_cpu:
