@@ -140,6 +140,79 @@ def adjoint_leg(tm, precision, nc=16, steps=200, ck=20, fd=False):
     return out
 
 
+def batched_leg(tm, steps=500, nrep=64, seed=2024, sequential=True):
+    """64 independent LJ boxes (cells per side uniform 6..24, density
+    U[0.70, 0.90], temperature U[0.7, 2.0]; counter-based generator, seed
+    2024), FP32-mixed, cutoff 2.5, skin 0.3, dt 0.005, reneighbour 20:
+    autotuned batched run (one hipGraph launch) against the same boxes run
+    one after another through the single-box path."""
+    boxes = tm.replica_boxes(seed=seed, nrep=nrep)
+    atoms = int(sum(len(b[1]) for b in boxes))
+    b = tm.Batch(boxes, cutoff=RC, skin=SKIN, dt=DT, reneigh=REN, precision="fp32mixed")
+    t0 = time.perf_counter()
+    cfg = b.autotune(trial_steps=2 * REN)
+    t_tune = time.perf_counter() - t0
+    b.run(2 * REN)  # graph capture for the timed length happens below; this warms the kernels
+    b.reset()
+    b.run(steps)    # captures the `steps` graph
+    b.reset()
+    t0 = time.perf_counter()
+    b.run(steps)
+    t1 = time.perf_counter()
+    gpu_ms = None
+    b.reset()
+    gpu_ms = b.run_timed(steps)
+    out = {"replicas": nrep, "atoms": atoms, "steps": steps, "config": cfg, "autotune_s": t_tune,
+           "batched_s": t1 - t0, "batched_atom_steps_per_s": atoms * steps / (t1 - t0),
+           "batched_graph_ms": gpu_ms}
+    b.close()
+    if sequential:
+        t_seq = 0.0
+        for aabb, x, v in boxes:
+            sim = tm.Simulation(np.ones(len(x)), x, v, aabb, RC + SKIN, precision="fp32mixed")
+            sim.copy_to_accelerator()
+            sim.setup(False, RC, SKIN)
+            sim.lennard_jones(False, RC)
+            sim.barrier()
+            s0 = time.perf_counter()
+            sim.run_lj(steps, REN, DT, False, RC, SKIN)
+            sim.barrier()
+            t_seq += time.perf_counter() - s0
+            sim.close()
+        out["sequential_s"] = t_seq
+        out["sequential_atom_steps_per_s"] = atoms * steps / t_seq
+        out["batched_over_sequential"] = t_seq / (t1 - t0)
+    return out
+
+
+def dem_leg(tm, steps=1000, nc=64):
+    """Granular gas: FCC 64^3 sites (a = 1.6796) kept where i + j < 2 nc
+    (528,384 spheres of diameter 1.0), velocities U[-1, 1], spring-dashpot
+    contact k = 100, damping 0.5 / 0.2, friction 0.3, contact cutoff 1.0,
+    skin 0.3, dt 0.001, half list, FP32-mixed, reneighbour 20."""
+    m, x, v = tm.generate_granular_gas(nc)
+    L = nc * 1.6796
+    sim = tm.Simulation(m, x, v, np.array([0.0, L, 0.0, L, 0.0, L]), 1.3, precision="fp32mixed", dem_radius=0.5)
+    sim.copy_to_accelerator()
+    sim.setup(True, 1.0, 0.3)
+    sim.dem(True, 1.0, 0.5, 0.2, 100.0, 0.3)
+    sim.run_dem(REN, REN, 0.001, True, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)  # warm-up
+    sim.barrier()
+    sim.profile(True)
+    sim.profile_reset()
+    t0 = time.perf_counter()
+    sim.run_dem(steps, REN, 0.001, True, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)
+    sim.barrier()
+    t1 = time.perf_counter()
+    ms, cnt = sim.profile_get("force")
+    sim.profile(False)
+    n = sim.nlocal
+    ke = sim.thermo(1.0)["KE"]
+    sim.close()
+    return {"atoms": n, "steps": steps, "seconds": t1 - t0, "atom_steps_per_s": n * steps / (t1 - t0),
+            "force_ms_avg": ms / max(cnt, 1), "kinetic_energy_end": ke}
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size (torch), for scaling the
     FETCH_SIZE / WRITE_SIZE counters of the force kernel."""
@@ -168,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--quick", action="store_true", help="headline variant only")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-trajectory leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
+    ap.add_argument("--no-dem", action="store_true", help="skip the DEM granular-gas leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -210,6 +285,15 @@ def main():
             adjoint[prec] = adjoint_leg(tm, prec, fd=not args.quick)
             print(f"# adjoint {prec}: {adjoint[prec]}", file=sys.stderr, flush=True)
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_leg(tm, sequential=not args.quick)
+        print(f"# batched: {batched}", file=sys.stderr, flush=True)
+    dem = None
+    if not args.no_dem:
+        dem = dem_leg(tm, steps=200 if args.quick else 1000)
+        print(f"# dem: {dem}", file=sys.stderr, flush=True)
+
     value = r32["atom_steps_per_s"]
     t_force = r32["force_ms_avg"] * 1e-3
     achieved = nbytes / t_force / 1e9
@@ -241,6 +325,10 @@ def main():
     }
     if adjoint:
         line["adjoint"] = adjoint
+    if batched:
+        line["batched"] = batched
+    if dem:
+        line["dem"] = dem
     if "fp64_full" in variants:
         line["fp32mixed_over_fp64"] = value / variants["fp64_full"]["atom_steps_per_s"]
     if not args.no_cpu:

@@ -114,6 +114,53 @@ int tmd_traj_adjoint(int steps, int reneigh_every, double dt, double cutoff, dou
                      double u_target, double p_target, double* out3, double* grad_theta2, struct Vector3D* grad_x0,
                      struct Vector3D* grad_v0);
 
+/* --- batched replicas (no counterpart in the reference) ---------------------
+ * Many independent periodic LJ boxes of different sizes advanced together:
+ * every kernel launch covers all replicas, and a whole run -- the main.cpp
+ * step loop with reneighbouring every `reneigh_every` steps -- is one
+ * hipGraph launch.  Full neighbour lists, unit masses, fresh ghosts.  Each
+ * replica's trajectory is bit-identical to the same box run alone through
+ * the md_* path with the same cell spacing, for any tile and block size.
+ * Handles are small positive integers; 0 means failure (tmd_last_error). */
+
+/* Replica parameters from a counter-based generator (SplitMix64 over
+ * (seed, replica, draw)): cells per side uniform in [cells_min, cells_max],
+ * density and temperature uniform in their ranges.  Host only. */
+int tmd_generate_replica_set(unsigned long long seed, int nrep, int cells_min, int cells_max, double rho_min,
+                             double rho_max, double temp_min, double temp_max, int* cells, double* rho, double* temp);
+/* counts[r] atoms per replica; pos / vel concatenated in replica order;
+ * aabbs: 6 per replica (xmin xmax ymin ymax zmin zmax). */
+int tmd_batch_create(int nrep, const int* counts, const struct Vector3D* pos, const struct Vector3D* vel,
+                     const double* aabbs);
+/* Parameters and the initial rebuild + force at the stored initial state.
+ * precision 0 FP64, 1 FP32-mixed, 2 FP32 Kahan. */
+int tmd_batch_setup(int handle, double cutoff, double skin, double epsilon, double sigma, double dt, int reneigh_every,
+                    int precision, int lj_shift);
+/* Launch configuration (restarts from the initial state): cells per tile
+ * edge (0 automatic per replica), threads per tile block (64..512, multiple
+ * of 64), cell edge / (cutoff + skin) (>= 1), list capacity headroom over
+ * the largest count, ghost room headroom over the expected ghost count. */
+int tmd_batch_configure(int handle, int tile, int threads, double spacing_factor, double list_headroom,
+                        double ghost_headroom);
+/* Times trial runs of the candidate configurations from the initial state,
+ * keeps the fastest and restarts from the initial state with it.  out
+ * (may be NULL): tile, threads, spacing factor, list headroom, ms/step. */
+int tmd_batch_autotune(int handle, int trial_steps, double* out5);
+/* Restart from the initial state with the current configuration. */
+int tmd_batch_reset(int handle);
+/* Advance all replicas `steps` steps (one graph launch); returns steps.
+ * The timed variant also returns the GPU time of the launch. */
+int tmd_batch_run(int handle, int steps);
+int tmd_batch_run_timed(int handle, int steps, double* gpu_ms);
+/* Locals of replica `rep` (any pointer may be NULL); returns their number. */
+int tmd_batch_get_state(int handle, int rep, struct Vector3D* pos, struct Vector3D* vel, struct Vector3D* force);
+/* Per replica 6 values as tmd_thermo: T, KE, PE, virial, P, pairs. */
+int tmd_batch_thermo(int handle, double* out);
+/* Per replica 8 ints: n, ghosts, cells per x, cells, tiles, tile edge, ghost
+ * room, list capacity. */
+int tmd_batch_info(int handle, int* out);
+void tmd_batch_destroy(int handle);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,23 +109,31 @@ __global__ void k_tile_fill(const int* __restrict__ start, const int* __restrict
 // Cells per tile: about 480 locals per tile (one 512-thread block), with
 // the window (tile + 1 cell on each side) inside the LDS budget.  An
 // explicit choice (tmd_set_tile) overrides.
-static void choose_tiles(Sim& s) {
-    const Geom& G = s.g;
-    int t[3] = {s.tile_req[0], s.tile_req[1], s.tile_req[2]};
-    if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) {
-        const double inner = (double)std::max(G.nx - 2, 1) * std::max(G.ny - 2, 1) * std::max(G.nz - 2, 1);
-        const double rho = std::max((double)s.n / inner, 1e-3);
-        int T = (int)std::lround(std::cbrt(480.0 / rho));
-        T = std::max(1, std::min(T, kMaxTile));
-        while (T > 1 && (T + 2.0) * (T + 2.0) * (T + 2.0) * rho * 28.0 > 0.85 * kLdsBudget) --T;
-        t[0] = t[1] = t[2] = T;
-    }
-    TileGeom& g = s.tiles;
+int auto_tile_edge(const Geom& G, int n) {
+    const double inner = (double)std::max(G.nx - 2, 1) * std::max(G.ny - 2, 1) * std::max(G.nz - 2, 1);
+    const double rho = std::max((double)n / inner, 1e-3);
+    int T = (int)std::lround(std::cbrt(480.0 / rho));
+    T = std::max(1, std::min(T, kMaxTile));
+    while (T > 1 && (T + 2.0) * (T + 2.0) * (T + 2.0) * rho * 28.0 > 0.85 * kLdsBudget) --T;
+    return T;
+}
+
+TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz) {
+    TileGeom g;
     g.nx = G.nx; g.ny = G.ny; g.nz = G.nz;
-    g.tx = std::min(t[0], kMaxTile); g.ty = std::min(t[1], kMaxTile); g.tz = std::min(t[2], kMaxTile);
+    g.tx = std::max(1, std::min(tx, kMaxTile));
+    g.ty = std::max(1, std::min(ty, kMaxTile));
+    g.tz = std::max(1, std::min(tz, kMaxTile));
     g.ntx = std::max(1, div_up(G.nx - 2, g.tx));
     g.nty = std::max(1, div_up(G.ny - 2, g.ty));
     g.ntz = std::max(1, div_up(G.nz - 2, g.tz));
+    return g;
+}
+
+static void choose_tiles(Sim& s) {
+    int t[3] = {s.tile_req[0], s.tile_req[1], s.tile_req[2]};
+    if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) t[0] = t[1] = t[2] = auto_tile_edge(s.g, s.n);
+    s.tiles = make_tile_geom(s.g, t[0], t[1], t[2]);
 }
 
 void bin_particles(Sim& s) {

@@ -28,167 +28,16 @@
 #include "tmd_internal.hpp"
 #include "tmd_window.hpp"
 #include "tmd_lj.hpp"
+#include "tmd_pair.hpp"
 
 namespace tmd {
 
 
 
-// error-free transformation: s + x = t + e exactly (Knuth TwoSum)
-__device__ __forceinline__ void two_sum_acc(float& s, float& c, float x) {
-    const float t = s + x;
-    const float bp = t - s;
-    const float ap = t - bp;
-    const float e = (s - ap) + (x - bp);
-    s = t;
-    c += e;
-}
-
-template <int SHIFT>
-__device__ __forceinline__ double lj_f64(const LJArgs& a, double rsq) {
-    const double sr2 = 1.0 / rsq;
-    const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
-    double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
-    if (SHIFT) f = f - a.fc - (rsq - a.rc2) * a.dfc;
-    return f;
-}
-
-template <int SHIFT>
-__device__ __forceinline__ float lj_f32(const LJArgs& a, float rsq) {
-    const float sr2 = __builtin_amdgcn_rcpf(rsq);
-    const float sr6 = sr2 * sr2 * sr2 * a.sigma6f;
-    float f = 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
-    if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
-    return f;
-}
-
-struct Nb8 {
-    int t[8];
-};
-
-__device__ __forceinline__ Nb8 unpack8(const int4 q) {
-    Nb8 r;
-    r.t[0] = q.x & 0xffff; r.t[1] = (unsigned)q.x >> 16;
-    r.t[2] = q.y & 0xffff; r.t[3] = (unsigned)q.y >> 16;
-    r.t[4] = q.z & 0xffff; r.t[5] = (unsigned)q.z >> 16;
-    r.t[6] = q.w & 0xffff; r.t[7] = (unsigned)q.w >> 16;
-    return r;
-}
-
 #define TMD_KERNEL_LIST_PARAMS                                                                                   \
     const double4 *__restrict__ pos, const double4 *__restrict__ spos, const int *__restrict__ start,              \
         const int *__restrict__ parts, const int *__restrict__ work, const int *__restrict__ tile_start,           \
         const int *__restrict__ nbc, const nb_t *__restrict__ nbl
-
-// FP64 pair scalar of the reference closure; UNIT skips the exact
-// multiplications by eps = 1 and sigma^6 = 1 (x * 1.0 == x), so the result
-// is bit-identical either way.
-template <int SHIFT, int UNIT>
-__device__ __forceinline__ double lj_f64u(const LJArgs& a, double rsq) {
-    const double sr2 = 1.0 / rsq;
-    const double sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6;
-    double f = UNIT ? 48.0 * sr6 * (sr6 - 0.5) * sr2 : 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
-    if (SHIFT) f = f - a.fc - (rsq - a.rc2) * a.dfc;
-    return f;
-}
-
-template <int SHIFT, int UNIT>
-__device__ __forceinline__ float lj_f32u(const LJArgs& a, float rsq) {
-    const float sr2 = __builtin_amdgcn_rcpf(rsq);
-    const float sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6f;
-    float f = UNIT ? 48.0f * sr6 * (sr6 - 0.5f) * sr2 : 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
-    if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
-    return f;
-}
-
-// Kahan compensated summation (s, c) += x
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
-// Neighbour position of window offset t: LDS (x, y, z interleaved) when the
-// window is staged, else the cell-ordered mirror.
-template <bool STAGED>
-__device__ __forceinline__ void win_pos(int t, const double* __restrict__ sp, const double4* __restrict__ spos, const TileWin& W,
-                                        double& x, double& y, double& z) {
-    if (STAGED) {
-        x = sp[3 * t + 0];
-        y = sp[3 * t + 1];
-        z = sp[3 * t + 2];
-    } else {
-        const double4 p = spos[win_index(t, W)];
-        x = p.x;
-        y = p.y;
-        z = p.z;
-    }
-}
-
-// Full-list force on one atom (lane).  Entries are read 8 at a time; the 8
-// pair terms are evaluated without branches (masked entries use a safe
-// rsq and a zero scalar) and accumulated in list order.  Adding a masked
-// term adds dx * 0.0 = +-0.0, which leaves the running sum unchanged (the
-// sum starts at +0.0 and never becomes -0.0), so the FP64 result equals the
-// sequential reference sum bit for bit.
-template <int PREC, int SHIFT, int UNIT, bool STAGED>
-__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const double* __restrict__ sp,
-                                                const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
-    const int ng = (cnt + 7) >> 3;
-    for (int gi = 0; gi < ng; ++gi) {
-        const Nb8 e = unpack8(lp[(size_t)gi * 64]);
-        const int k0 = gi * 8;
-        double dx[8], dy[8], dz[8], rs[8];
-        bool ok[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const bool valid = k0 + u < cnt;
-            double xj, yj, zj;
-            win_pos<STAGED>(valid ? e.t[u] : 0, sp, spos, W, xj, yj, zj);
-            dx[u] = pi.x - xj;
-            dy[u] = pi.y - yj;
-            dz[u] = pi.z - zj;
-            const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-            ok[u] = valid && rsq < a.rc2;
-            rs[u] = ok[u] ? rsq : 1.0;
-        }
-        if (PREC == PREC_FP64) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double f = lj_f64u<SHIFT, UNIT>(a, rs[u]);
-                const double fm = ok[u] ? f : 0.0;
-                fx = fx + dx[u] * fm;
-                fy = fy + dy[u] * fm;
-                fz = fz + dz[u] * fm;
-            }
-        } else if (PREC == PREC_FP32_MIXED) {
-            // FP32 pair scalar, FP64 accumulation of dx * f (each FP32 term is
-            // exact in FP64; the 29 extra mantissa bits absorb the sum's rounding)
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
-                const double fm = (double)(ok[u] ? f : 0.0f);
-                fx = fx + dx[u] * fm;
-                fy = fy + dy[u] * fm;
-                fz = fz + dz[u] * fm;
-            }
-        } else {
-            // FP32 pair terms with Kahan-compensated FP32 accumulation
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
-                const float fm = ok[u] ? f : 0.0f;
-                kahan_add(sxa, cxa, (float)dx[u] * fm);
-                kahan_add(sya, cya, (float)dy[u] * fm);
-                kahan_add(sza, cza, (float)dz[u] * fm);
-            }
-        }
-    }
-    if (PREC != PREC_FP32_KAHAN) return make_double4(fx, fy, fz, 0.0);
-    return make_double4((double)sxa + (double)cxa, (double)sya + (double)cya, (double)sza + (double)cza, 0.0);
-}
 
 template <int PREC, int SHIFT, int UNIT>
 __global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force) {
@@ -316,40 +165,7 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARA
     const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
-        const int i = work[w];
-        const double4 pi = pos[i];
-        const int cnt = nbc[w];
-        for (int k = 0; k < cnt; ++k) {
-            const int t = nbl[nb16_slot(w, k, a.cap)];
-            int j;
-            double xj, yj, zj;
-            if (staged) {
-                j = sj[t];
-                xj = sx[t]; yj = sy[t]; zj = sz[t];
-            } else {
-                const int kk = win_index(t, W);
-                j = parts[kk];
-                const double4 p = spos[kk];
-                xj = p.x; yj = p.y; zj = p.z;
-            }
-            const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            if (rsq < a.rc2) {
-                const double sr2 = 1.0 / rsq;
-                const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
-                double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
-                double u = 4.0 * a.eps * sr6 * (sr6 - 1.0);
-                if (SHIFT) {
-                    const double ds = rsq - a.rc2;
-                    f = f - a.fc - ds * a.dfc;
-                    u = u - a.uc + ds * (0.5 * a.fc) + ds * ds * (0.25 * a.dfc);
-                }
-                const double wt = half ? (j < a.n ? 1.0 : 0.5) : 0.5;
-                pe += wt * u;
-                vir += wt * (f * rsq);
-                np += 1.0;
-            }
-        }
+        lj_energy_atom<SHIFT>(pos[work[w]], w, nbc[w], nbl, a, half, a.n, staged, sx, sy, sz, sj, spos, parts, W, pe, vir, np);
     }
     const double bpe = block_sum_double<kTileThreads>(pe);
     const double bvir = block_sum_double<kTileThreads>(vir);

@@ -11,6 +11,7 @@
 // the reference.  Half lists keep j < i among locals (the reference's
 // filter) plus, by default, every ghost.
 #include "tmd_internal.hpp"
+#include "tmd_pair.hpp"
 
 namespace tmd {
 
@@ -21,12 +22,6 @@ __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict_
     if (k >= *nbinned) return;
     spos[k] = pos[parts[k]];
 }
-
-struct NbArgs {
-    int n, cap, half, half_mode, wcap;
-    double cut2;
-    TileGeom g;
-};
 
 __global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restrict__ spos, const double4* __restrict__ pos,
                                                         const int* __restrict__ cell_of, const int* __restrict__ start,
@@ -45,49 +40,10 @@ __global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restric
     double* sz = dyn + 2 * a.wcap;
     int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
     const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
-
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int c = cell_of[i];
-        const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
-        const double4 pi = pos[i];
-        int cnt = 0;
-        for (int r = 0; r < 9; ++r) {
-            int t0, t1;
-            win_stencil_row(cx, cy, cz, r % 3 - 1, r / 3 - 1, start, a.g, W, t0, t1);
-            // 4 candidates per pass: loads and distances first (independent),
-            // then in-order appends
-            for (int tb = t0; tb < t1; tb += 4) {
-                int jj[4];
-                bool ok[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int t = min(tb + u, t1 - 1);
-                    double xj, yj, zj;
-                    if (staged) {
-                        jj[u] = sj[t];
-                        xj = sx[t]; yj = sy[t]; zj = sz[t];
-                    } else {
-                        const int kk = win_index(t, W);
-                        jj[u] = parts[kk];
-                        const double4 p = spos[kk];
-                        xj = p.x; yj = p.y; zj = p.z;
-                    }
-                    const int j = jj[u];
-                    bool keep = (tb + u < t1) && (j != i);
-                    if (a.half) keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
-                    const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-                    const double rsq = dx * dx + dy * dy + dz * dz;
-                    ok[u] = keep && rsq <= a.cut2;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (ok[u] && cnt < a.cap) nb_list[nb16_slot(w, cnt, a.cap)] = (nb_t)(tb + u);
-                    cnt += ok[u] ? 1 : 0;
-                }
-            }
-        }
+        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], start, a, W, staged, sx, sy, sz, sj, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(overflow, cnt);
     }

@@ -59,6 +59,14 @@ void scan_tiles(Sim& s, int* tiles, int ntiles, int* total_dev) {
     TMD_HIP(hipGetLastError());
 }
 
+void scan_exclusive(const int* in, int* out, int n, int* total_dev, int* tiles, hipStream_t st) {
+    const int ntiles = div_up(n, kScanTile);
+    hipLaunchKernelGGL(k_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, st, in, n, tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, st, tiles, ntiles, total_dev);
+    hipLaunchKernelGGL(k_scan_finish, dim3(ntiles), dim3(kScanThreads), 0, st, in, n, (const int*)tiles, out);
+    TMD_HIP(hipGetLastError());
+}
+
 void exclusive_scan_int(const int* in, int* out, int n, int* total_dev, Sim& s) {
     if (n <= 0) {
         if (total_dev) TMD_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), s.stream));

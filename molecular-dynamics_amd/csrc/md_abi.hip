@@ -201,7 +201,10 @@ static bool ready(Sim& s) {
 }
 
 static void set_geometry(Sim& s, const double* aabb, const double* rank_aabb, double cell_spacing) {
-    Geom& G = s.g;
+    make_geom(s.g, aabb, rank_aabb, cell_spacing);
+}
+
+void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell_spacing) {
     for (int d = 0; d < 6; ++d) G.world[d] = aabb[d];
     for (int d = 0; d < 3; ++d) {
         G.aabb[2 * d] = rank_aabb[2 * d] - cell_spacing;

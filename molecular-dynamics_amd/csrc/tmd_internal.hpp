@@ -146,13 +146,21 @@ inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 // scan / compaction
 void exclusive_scan_int(const int* in, int* out, int n, int* total_dev, Sim& s);
 void scan_tiles(Sim& s, int* tiles, int ntiles, int* total_dev);  // one-block exclusive scan in place
+// exclusive scan of n > 0 ints with caller scratch of div_up(n, kScanTile)
+// ints; no allocation, no host synchronisation (graph-capturable)
+void scan_exclusive(const int* in, int* out, int n, int* total_dev, int* tiles, hipStream_t st);
 
 // borders / ghosts
 void build_ghosts(Sim& s);
 void refresh_ghosts(Sim& s);
 
+// geometry (core/grid.impala:94-98): world AABB, rank AABB, cell spacing
+void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell_spacing);
+
 // binning
 void bin_particles(Sim& s);
+int auto_tile_edge(const Geom& G, int n);  // cells per tile edge for ~480 locals per tile
+TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz);
 
 // tile windows (after binning), position mirror, neighbour lists
 void build_windows(Sim& s);

@@ -1,0 +1,251 @@
+// Per-atom pair-force bodies shared by the single-box kernels (k_force.hip)
+// and the batched-replica kernels (k_batch.hip): both evaluate exactly the
+// same arithmetic in the same order.
+#pragma once
+
+#include "tmd_window.hpp"
+#include "tmd_lj.hpp"
+
+namespace tmd {
+
+template <int SHIFT>
+__device__ __forceinline__ double lj_f64(const LJArgs& a, double rsq) {
+    const double sr2 = 1.0 / rsq;
+    const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
+    double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
+    if (SHIFT) f = f - a.fc - (rsq - a.rc2) * a.dfc;
+    return f;
+}
+
+template <int SHIFT>
+__device__ __forceinline__ float lj_f32(const LJArgs& a, float rsq) {
+    const float sr2 = __builtin_amdgcn_rcpf(rsq);
+    const float sr6 = sr2 * sr2 * sr2 * a.sigma6f;
+    float f = 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
+    if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
+    return f;
+}
+
+struct Nb8 {
+    int t[8];
+};
+
+__device__ __forceinline__ Nb8 unpack8(const int4 q) {
+    Nb8 r;
+    r.t[0] = q.x & 0xffff; r.t[1] = (unsigned)q.x >> 16;
+    r.t[2] = q.y & 0xffff; r.t[3] = (unsigned)q.y >> 16;
+    r.t[4] = q.z & 0xffff; r.t[5] = (unsigned)q.z >> 16;
+    r.t[6] = q.w & 0xffff; r.t[7] = (unsigned)q.w >> 16;
+    return r;
+}
+
+
+// FP64 pair scalar of the reference closure; UNIT skips the exact
+// multiplications by eps = 1 and sigma^6 = 1 (x * 1.0 == x), so the result
+// is bit-identical either way.
+template <int SHIFT, int UNIT>
+__device__ __forceinline__ double lj_f64u(const LJArgs& a, double rsq) {
+    const double sr2 = 1.0 / rsq;
+    const double sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6;
+    double f = UNIT ? 48.0 * sr6 * (sr6 - 0.5) * sr2 : 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
+    if (SHIFT) f = f - a.fc - (rsq - a.rc2) * a.dfc;
+    return f;
+}
+
+template <int SHIFT, int UNIT>
+__device__ __forceinline__ float lj_f32u(const LJArgs& a, float rsq) {
+    const float sr2 = __builtin_amdgcn_rcpf(rsq);
+    const float sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6f;
+    float f = UNIT ? 48.0f * sr6 * (sr6 - 0.5f) * sr2 : 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
+    if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
+    return f;
+}
+
+// Kahan compensated summation (s, c) += x
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Neighbour position of window offset t: LDS (x, y, z interleaved) when the
+// window is staged, else the cell-ordered mirror.
+template <bool STAGED>
+__device__ __forceinline__ void win_pos(int t, const double* __restrict__ sp, const double4* __restrict__ spos, const TileWin& W,
+                                        double& x, double& y, double& z) {
+    if (STAGED) {
+        x = sp[3 * t + 0];
+        y = sp[3 * t + 1];
+        z = sp[3 * t + 2];
+    } else {
+        const double4 p = spos[win_index(t, W)];
+        x = p.x;
+        y = p.y;
+        z = p.z;
+    }
+}
+
+// Full-list force on one atom (lane).  Entries are read 8 at a time; the 8
+// pair terms are evaluated without branches (masked entries use a safe
+// rsq and a zero scalar) and accumulated in list order.  Adding a masked
+// term adds dx * 0.0 = +-0.0, which leaves the running sum unchanged (the
+// sum starts at +0.0 and never becomes -0.0), so the FP64 result equals the
+// sequential reference sum bit for bit.
+template <int PREC, int SHIFT, int UNIT, bool STAGED>
+__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const double* __restrict__ sp,
+                                                const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
+    const int ng = (cnt + 7) >> 3;
+    for (int gi = 0; gi < ng; ++gi) {
+        const Nb8 e = unpack8(lp[(size_t)gi * 64]);
+        const int k0 = gi * 8;
+        double dx[8], dy[8], dz[8], rs[8];
+        bool ok[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool valid = k0 + u < cnt;
+            double xj, yj, zj;
+            win_pos<STAGED>(valid ? e.t[u] : 0, sp, spos, W, xj, yj, zj);
+            dx[u] = pi.x - xj;
+            dy[u] = pi.y - yj;
+            dz[u] = pi.z - zj;
+            const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            ok[u] = valid && rsq < a.rc2;
+            rs[u] = ok[u] ? rsq : 1.0;
+        }
+        if (PREC == PREC_FP64) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double f = lj_f64u<SHIFT, UNIT>(a, rs[u]);
+                const double fm = ok[u] ? f : 0.0;
+                fx = fx + dx[u] * fm;
+                fy = fy + dy[u] * fm;
+                fz = fz + dz[u] * fm;
+            }
+        } else if (PREC == PREC_FP32_MIXED) {
+            // FP32 pair scalar, FP64 accumulation of dx * f (each FP32 term is
+            // exact in FP64; the 29 extra mantissa bits absorb the sum's rounding)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
+                const double fm = (double)(ok[u] ? f : 0.0f);
+                fx = fx + dx[u] * fm;
+                fy = fy + dy[u] * fm;
+                fz = fz + dz[u] * fm;
+            }
+        } else {
+            // FP32 pair terms with Kahan-compensated FP32 accumulation
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
+                const float fm = ok[u] ? f : 0.0f;
+                kahan_add(sxa, cxa, (float)dx[u] * fm);
+                kahan_add(sya, cya, (float)dy[u] * fm);
+                kahan_add(sza, cza, (float)dz[u] * fm);
+            }
+        }
+    }
+    if (PREC != PREC_FP32_KAHAN) return make_double4(fx, fy, fz, 0.0);
+    return make_double4((double)sxa + (double)cxa, (double)sya + (double)cya, (double)sza + (double)cza, 0.0);
+}
+
+
+// Energy and virial terms of one atom's list entries (FP64): U = 4 eps
+// (sr12 - sr6) [+ shift terms], W = f r^2.  Full lists weight every entry
+// 1/2; half lists weight local-local entries 1 and local-ghost entries 1/2
+// (an index j >= nloc_end is a ghost).
+template <int SHIFT>
+__device__ __forceinline__ void lj_energy_atom(const double4 pi, int w, int cnt, const nb_t* __restrict__ nbl, const LJArgs& a,
+                                               int half, int nloc_end, bool staged, const double* sx, const double* sy,
+                                               const double* sz, const int* sj, const double4* __restrict__ spos,
+                                               const int* __restrict__ parts, const TileWin& W, double& pe, double& vir,
+                                               double& np) {
+    for (int k = 0; k < cnt; ++k) {
+        const int t = nbl[nb16_slot(w, k, a.cap)];
+        int j;
+        double xj, yj, zj;
+        if (staged) {
+            j = sj[t];
+            xj = sx[t]; yj = sy[t]; zj = sz[t];
+        } else {
+            const int kk = win_index(t, W);
+            j = parts[kk];
+            const double4 p = spos[kk];
+            xj = p.x; yj = p.y; zj = p.z;
+        }
+        const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        if (rsq < a.rc2) {
+            const double sr2 = 1.0 / rsq;
+            const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
+            double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
+            double u = 4.0 * a.eps * sr6 * (sr6 - 1.0);
+            if (SHIFT) {
+                const double ds = rsq - a.rc2;
+                f = f - a.fc - ds * a.dfc;
+                u = u - a.uc + ds * (0.5 * a.fc) + ds * ds * (0.25 * a.dfc);
+            }
+            const double wt = half ? (j < nloc_end ? 1.0 : 0.5) : 0.5;
+            pe += wt * u;
+            vir += wt * (f * rsq);
+            np += 1.0;
+        }
+    }
+}
+
+struct NbArgs {
+    int n, cap, half, half_mode, wcap;  // n: first ghost index (locals are below it)
+    double cut2;
+    TileGeom g;
+};
+
+// Verlet list of local atom i (work slot w, cell c of the tile's grid) from
+// the staged window: 9 stencil rows of 3 cells in the reference's order,
+// `rsq <= cut2` in FP64 without contraction, u16 window offsets.  Returns
+// the count (entries beyond the capacity are counted, not stored).
+__device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi, const int* __restrict__ start,
+                                          const NbArgs& a, const TileWin& W, bool staged, const double* sx, const double* sy,
+                                          const double* sz, const int* sj, const double4* __restrict__ spos,
+                                          const int* __restrict__ parts, nb_t* __restrict__ nb_list) {
+    const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
+    int cnt = 0;
+    for (int r = 0; r < 9; ++r) {
+        int t0, t1;
+        win_stencil_row(cx, cy, cz, r % 3 - 1, r / 3 - 1, start, a.g, W, t0, t1);
+        // 4 candidates per pass: loads and distances first (independent),
+        // then in-order appends
+        for (int tb = t0; tb < t1; tb += 4) {
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = min(tb + u, t1 - 1);
+                int j;
+                double xj, yj, zj;
+                if (staged) {
+                    j = sj[t];
+                    xj = sx[t]; yj = sy[t]; zj = sz[t];
+                } else {
+                    const int kk = win_index(t, W);
+                    j = parts[kk];
+                    const double4 p = spos[kk];
+                    xj = p.x; yj = p.y; zj = p.z;
+                }
+                bool keep = (tb + u < t1) && (j != i);
+                if (a.half) keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                ok[u] = keep && rsq <= a.cut2;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (ok[u] && cnt < a.cap) nb_list[nb16_slot(w, cnt, a.cap)] = (nb_t)(tb + u);
+                cnt += ok[u] ? 1 : 0;
+            }
+        }
+    }
+    return cnt;
+}
+
+}  // namespace tmd

@@ -58,8 +58,8 @@ __device__ __forceinline__ void tile_window(int tile, const int* __restrict__ st
     const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
     const int wy = g.ty + 2;
     const int nseg = wy * (g.tz + 2);
-    if (t < nseg) {
-        const int y = cy0 - 1 + t % wy, z = cz0 - 1 + t / wy;
+    for (int s = t; s < nseg; s += blockDim.x) {
+        const int y = cy0 - 1 + s % wy, z = cz0 - 1 + s / wy;
         const int xlo = max(cx0 - 1, 0), xhi = min(cx0 + g.tx, g.nx - 1);
         int b = 0, len = 0;
         if (y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
@@ -67,8 +67,8 @@ __device__ __forceinline__ void tile_window(int tile, const int* __restrict__ st
             b = start[row + xlo];
             len = start[row + xhi + 1] - b;
         }
-        W.seg_beg[t] = b;
-        W.seg_off[t + 1] = len;
+        W.seg_beg[s] = b;
+        W.seg_off[s + 1] = len;
     }
     if (t == 0) {
         W.cx0 = cx0;

@@ -91,6 +91,18 @@ SIGNATURES = {
     "tmd_profile_reset": (None, []),
     "tmd_generate_fcc": (_i, [_i, _i, _i, _d, _i, _dp, _vp, _vp, _i]),
     "tmd_generate_granular_gas": (_i, [_i, _d, ctypes.c_ulonglong, _d, _dp, _vp, _vp, _i]),
+    "tmd_generate_replica_set": (_i, [ctypes.c_ulonglong, _i, _i, _i, _d, _d, _d, _d, _ip, _dp, _dp]),
+    "tmd_batch_create": (_i, [_i, _ip, _vp, _vp, _dp]),
+    "tmd_batch_setup": (_i, [_i, _d, _d, _d, _d, _d, _i, _i, _i]),
+    "tmd_batch_configure": (_i, [_i, _i, _i, _d, _d, _d]),
+    "tmd_batch_autotune": (_i, [_i, _i, _dp]),
+    "tmd_batch_reset": (_i, [_i]),
+    "tmd_batch_run": (_i, [_i, _i]),
+    "tmd_batch_run_timed": (_i, [_i, _i, _dp]),
+    "tmd_batch_get_state": (_i, [_i, _i, _vp, _vp, _vp]),
+    "tmd_batch_thermo": (_i, [_i, _dp]),
+    "tmd_batch_info": (_i, [_i, _ip]),
+    "tmd_batch_destroy": (None, [_i]),
     "tmd_traj_loss": (_i, [_i, _i, _d, _d, _d, _d, _d, _dp]),
     "tmd_traj_adjoint": (_i, [_i, _i, _d, _d, _d, _d, _d, _d, _d, _dp, _dp, _vp, _vp]),
 }

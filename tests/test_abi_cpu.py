@@ -127,3 +127,27 @@ def test_host_lattice_generators_match_oracle():
     assert np.all(np.abs(v) <= 1.0) and abs(v.mean()) < 0.05
     m64, _, _ = tm.generate_granular_gas(64)
     assert len(m64) == 528384
+
+
+def test_replica_set_generator():
+    """Counter-based replica parameters: deterministic, in range, and replica
+    r's draw does not depend on how many replicas are asked for."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+    import numpy as np
+    import tinymd_amd as tm
+    c, r, t = tm.replica_set(seed=2024, nrep=64)
+    assert c.min() >= 6 and c.max() <= 24
+    assert np.all((r >= 0.70) & (r < 0.90)) and np.all((t >= 0.7) & (t < 2.0))
+    c5, r5, t5 = tm.replica_set(seed=2024, nrep=5)
+    assert np.array_equal(c[:5], c5) and np.array_equal(r[:5], r5) and np.array_equal(t[:5], t5)
+    c2, _, _ = tm.replica_set(seed=2025, nrep=64)
+    assert not np.array_equal(c, c2)
+    # the 64-replica workload: sum of 4 nc^3 atoms
+    assert int((4 * c.astype(np.int64) ** 3).sum()) == 1334984
+    boxes = tm.replica_boxes(seed=2024, nrep=3)
+    for (aabb, x, v), nc, rho, temp in zip(boxes, c, r, t):
+        assert len(x) == 4 * nc ** 3
+        assert aabb[1] == pytest.approx(nc * (4.0 / rho) ** (1.0 / 3.0))
+        assert np.abs(v.mean(axis=0)).max() < 1e-12
+        assert np.sum(v * v) / (3 * len(v) - 3) == pytest.approx(temp, rel=1e-12)
