@@ -200,6 +200,7 @@ static void snapshot_to(Sim& s, DevBuf& bx, DevBuf& bv, size_t slot) {
 }
 
 static void restore_from(Sim& s, const DevBuf& bx, const DevBuf& bv, size_t slot) {
+    s.mirror_fresh = 0;
     copy_state(s, s.pos.as<double4>(), s.vel.as<double4>(), bx.as<double4>() + slot * s.n, bv.as<double4>() + slot * s.n);
 }
 
@@ -231,9 +232,9 @@ static void accumulate_thermo(Sim& s, double rc, double eps, double sigma, doubl
 
 static void launch_lj_adjoint(Sim& s, double rc, double eps, double sigma, const AdjCoef& c, double* theta) {
     LJArgs a = make_lj_args(s, rc, eps, sigma);
-    refresh_mirror(s);
+    if (!s.mirror_fresh) refresh_mirror(s);
     const int ntiles = ntiles_of(s);
-    const size_t lds = (size_t)a.wcap * (3 * sizeof(double) + sizeof(int));
+    const size_t lds = (size_t)a.wcap * kStageBytes;
     double* part = partials(s, 2 * (size_t)ntiles);
 #define TMD_ADJ(P, SH)                                                                                                  \
     hipLaunchKernelGGL((k_lj_adjoint<P, SH>), dim3(ntiles), dim3(kTileThreads), lds, s.stream,                          \
@@ -280,8 +281,7 @@ static void forward_traj(Sim& s, const TrajSpec& p, bool store_ckpt, double* uba
         } else {
             refresh_ghosts(s);
         }
-        compute_lj(s, 0, p.rc, p.eps, p.sigma);
-        final_integration(s, p.dt);
+        compute_lj(s, 0, p.rc, p.eps, p.sigma, p.dt);  // final kick fused
         accumulate_thermo(s, p.rc, p.eps, p.sigma, acc);
         if (store_ckpt && t % p.ck == 0) snapshot_to(s, s.adj_ckx, s.adj_ckv, (size_t)(t / p.ck));
     }
@@ -389,8 +389,7 @@ bool traj_adjoint(Sim& s, int steps, int ck, double dt, double rc, double skin, 
         for (int t = t0 + 1; t <= t1 && !has_error(); ++t) {
             initial_integration(s, dt);
             refresh_ghosts(s);
-            compute_lj(s, 0, rc, eps, sigma);
-            final_integration(s, dt);
+            compute_lj(s, 0, rc, eps, sigma, dt);  // final kick fused
             snapshot_to(s, s.adj_segx, s.adj_segv, (size_t)(t - t0));
         }
         // reverse it

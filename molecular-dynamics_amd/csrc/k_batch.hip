@@ -92,7 +92,8 @@ __global__ void k_b_fill(int* __restrict__ p, int n, int v) {
 // ---------------------------------------------------------------------------
 // integration and periodic wrap (k_integrate.hip arithmetic)
 __global__ void k_b_int1(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, double4* __restrict__ pos,
-                         double4* __restrict__ vel, const double4* __restrict__ force, double hdt, double dt) {
+                         double4* __restrict__ vel, const double4* __restrict__ force, double hdt, double dt,
+                         const int* __restrict__ inv, double4* __restrict__ spos) {
     const int r = eb_rep[blockIdx.x];
     const int i = (blockIdx.x - R[r].eb0) * kBlk + threadIdx.x;
     if (i >= R[r].n) return;
@@ -108,6 +109,8 @@ __global__ void k_b_int1(const RepDesc* __restrict__ R, const int* __restrict__ 
     x.z = x.z + v.z * dt;
     vel[s] = v;
     pos[s] = x;
+    const int k = inv[s];  // keep the cell-ordered mirror current
+    if (k >= 0) spos[k] = x;
 }
 
 __global__ void k_b_int2(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, double4* __restrict__ vel,
@@ -268,7 +271,8 @@ __global__ void __launch_bounds__(kScanThreads) k_b_border_scatter(const RepDesc
 
 // per-step ghost refresh: ghost = owner + image * L
 __global__ void k_b_refresh(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep, const int* __restrict__ dyn,
-                            double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code) {
+                            double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
+                            const int* __restrict__ inv, double4* __restrict__ spos) {
     const int r = gb_rep[blockIdx.x];
     const RepDesc& D = R[r];
     const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
@@ -283,6 +287,8 @@ __global__ void k_b_refresh(const RepDesc* __restrict__ R, const int* __restrict
     q.z = p.z + D.len[2] * (double)iz;
     q.w = p.w;
     pos[s] = q;
+    const int k = inv[s];
+    if (k >= 0) spos[k] = q;
 }
 
 // ---------------------------------------------------------------------------
@@ -290,7 +296,7 @@ __global__ void k_b_refresh(const RepDesc* __restrict__ R, const int* __restrict
 // counters and the CSR are global
 __global__ void k_b_bin_count(const RepDesc* __restrict__ R, const int* __restrict__ cb_rep, const int* __restrict__ dyn,
                               const double4* __restrict__ pos, int* __restrict__ cell_of, int* __restrict__ slot,
-                              int* __restrict__ count) {
+                              int* __restrict__ count, int* __restrict__ inv) {
     const int r = cb_rep[blockIdx.x];
     const RepDesc& D = R[r];
     const int i = (blockIdx.x - D.cb0) * kBlk + threadIdx.x;
@@ -307,6 +313,7 @@ __global__ void k_b_bin_count(const RepDesc* __restrict__ R, const int* __restri
     }
     cell_of[D.a0 + i] = c;
     slot[D.a0 + i] = (c >= 0) ? atomicAdd(&count[D.c0 + c], 1) : -1;
+    inv[D.a0 + i] = -1;  // set by k_b_bin_sort for binned atoms
 }
 
 __global__ void k_b_bin_scatter(const RepDesc* __restrict__ R, const int* __restrict__ cb_rep, const int* __restrict__ dyn,
@@ -321,7 +328,7 @@ __global__ void k_b_bin_scatter(const RepDesc* __restrict__ R, const int* __rest
 }
 
 __global__ void k_b_bin_sort(const RepDesc* __restrict__ R, const int* __restrict__ cell_rep, const int* __restrict__ start,
-                             int ncells_all, int* __restrict__ parts, int* __restrict__ local_count) {
+                             int ncells_all, int* __restrict__ parts, int* __restrict__ local_count, int* __restrict__ inv) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells_all) return;
     const int lim = R[cell_rep[c]].a0 + R[cell_rep[c]].n;
@@ -338,6 +345,7 @@ __global__ void k_b_bin_sort(const RepDesc* __restrict__ R, const int* __restric
     int nl = 0;
     while (b + nl < e && parts[b + nl] < lim) ++nl;
     local_count[c] = nl;
+    for (int k = b; k < e; ++k) inv[parts[k]] = k;
 }
 
 __global__ void k_b_tile_stats(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep, const int* __restrict__ start,
@@ -432,15 +440,12 @@ __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restr
     const int* st = start + D.c0;
     __shared__ TileWin W;
     tile_window(tile - D.t0, st, D.g, W);
-    double* sx = dyn;
-    double* sy = dyn + a.wcap;
-    double* sz = dyn + 2 * a.wcap;
-    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
-    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    double4* sp = reinterpret_cast<double4*>(dyn);
+    const bool staged = stage_tile_xyzj(W, a.wcap, spos, parts, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], st, aa, W, staged, sx, sy, sz, sj, spos, parts, nb_list);
+        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], st, aa, W, staged, sp, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(&flags[FL_LIST], cnt);
     }
@@ -451,23 +456,32 @@ __global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __res
                                                             const double4* __restrict__ pos, const double4* __restrict__ spos,
                                                             const int* __restrict__ start, const int* __restrict__ work,
                                                             const int* __restrict__ tile_start, const int* __restrict__ nbc,
-                                                            const nb_t* __restrict__ nbl, LJArgs a, double4* __restrict__ force) {
+                                                            const nb_t* __restrict__ nbl, LJArgs a, double4* __restrict__ force,
+                                                            double4* __restrict__ vel, double hdt) {
     extern __shared__ double dyn[];
+    double4* sp = reinterpret_cast<double4*>(dyn);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
     const RepDesc& D = R[tile_rep[tile]];
     __shared__ TileWin W;
     tile_window(tile - D.t0, start + D.c0, D.g, W);
-    const bool staged = stage_tile_aos(W, a.wcap, spos, dyn);
+    const bool staged = stage_tile_aos(W, a.wcap, spos, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
         const double4 pi = pos[i];
         const int cnt = nbc[w];
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
-        force[i] = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, dyn, spos, W, a)
-                          : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, dyn, spos, W, a);
+        const double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
+                                 : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
+        force[i] = f;
+        // velocity-Verlet final kick (k_final_integration's arithmetic)
+        double4 v = vel[i];
+        v.x = v.x + f.x * hdt;
+        v.y = v.y + f.y * hdt;
+        v.z = v.z + f.z * hdt;
+        vel[i] = v;
     }
 }
 
@@ -566,7 +580,7 @@ struct Batch {
     int n_sb = 0, n_eb = 0, n_cb = 0, n_gb = 0, n_tiles = 0, n_cells = 0, n_slots = 0, n_work = 0;
     int list_cap = 8, wcap = 0;
     DevBuf dR, sb_rep, eb_rep, cb_rep, gb_rep, tile_rep, cell_rep;
-    DevBuf pos, vel, force, g_owner, g_code, cell_of, cell_slot, parts, spos;
+    DevBuf pos, vel, force, g_owner, g_code, cell_of, cell_slot, cell_inv, parts, spos;
     DevBuf cell_count, cell_start, tile_count, tile_start, work;
     DevBuf sb_cnt0, sb_cnt1, scan_tmp, nb_count, nb_list;
     DevBuf dyn, flags, epart, kpart, tout;
@@ -578,7 +592,7 @@ struct Batch {
     ~Batch() {
         drop_graphs();
         DevBuf* bufs[] = {&dR, &sb_rep, &eb_rep, &cb_rep, &gb_rep, &tile_rep, &cell_rep, &pos, &vel, &force, &g_owner,
-                          &g_code, &cell_of, &cell_slot, &parts, &spos, &cell_count, &cell_start, &tile_count,
+                          &g_code, &cell_of, &cell_slot, &cell_inv, &parts, &spos, &cell_count, &cell_start, &tile_count,
                           &tile_start, &work, &sb_cnt0, &sb_cnt1, &scan_tmp, &nb_count, &nb_list, &dyn, &flags, &epart,
                           &kpart, &tout, &snap_pos, &snap_vel, &snap_force};
         if (stream) (void)hipStreamSynchronize(stream);
@@ -697,6 +711,7 @@ static void layout(Batch& B) {
     B.g_code.reserve(sizeof(int) * S, false, st);
     B.cell_of.reserve(sizeof(int) * S, false, st);
     B.cell_slot.reserve(sizeof(int) * S, false, st);
+    B.cell_inv.reserve(sizeof(int) * S, false, st);
     B.parts.reserve(sizeof(int) * S, false, st);
     B.spos.reserve(sizeof(double4) * S, false, st);
     B.cell_count.reserve(sizeof(int) * (size_t)B.n_cells, false, st);
@@ -766,14 +781,15 @@ static void enq_rebuild(Batch& B) {
     fill(B, B.cell_count, B.n_cells, 0);
     hipLaunchKernelGGL(k_b_bin_count, dim3(B.n_cb), dim3(kBlk), 0, st, R, (const int*)B.cb_rep.as<int>(),
                        (const int*)B.dyn.as<int>(), (const double4*)B.pos.as<double4>(), B.cell_of.as<int>(),
-                       B.cell_slot.as<int>(), B.cell_count.as<int>());
+                       B.cell_slot.as<int>(), B.cell_count.as<int>(), B.cell_inv.as<int>());
     scan_exclusive(B.cell_count.as<int>(), B.cell_start.as<int>(), B.n_cells, B.cell_start.as<int>() + B.n_cells,
                    B.scan_tmp.as<int>(), st);
     hipLaunchKernelGGL(k_b_bin_scatter, dim3(B.n_cb), dim3(kBlk), 0, st, R, (const int*)B.cb_rep.as<int>(),
                        (const int*)B.dyn.as<int>(), (const int*)B.cell_of.as<int>(), (const int*)B.cell_slot.as<int>(),
                        (const int*)B.cell_start.as<int>(), B.parts.as<int>());
     hipLaunchKernelGGL(k_b_bin_sort, dim3(div_up(B.n_cells, kBlk)), dim3(kBlk), 0, st, R, (const int*)B.cell_rep.as<int>(),
-                       (const int*)B.cell_start.as<int>(), B.n_cells, B.parts.as<int>(), B.cell_count.as<int>());
+                       (const int*)B.cell_start.as<int>(), B.n_cells, B.parts.as<int>(), B.cell_count.as<int>(),
+                       B.cell_inv.as<int>());
     hipLaunchKernelGGL(k_b_tile_stats, dim3(div_up(B.n_tiles, kBlk)), dim3(kBlk), 0, st, R, (const int*)B.tile_rep.as<int>(),
                        (const int*)B.cell_start.as<int>(), (const int*)B.cell_count.as<int>(), B.n_tiles,
                        B.tile_count.as<int>(), B.flags.as<int>());
@@ -801,7 +817,7 @@ static void enq_nlist(Batch& B) {
     a.wcap = B.wcap;
     const double cut = B.rc + B.skin;
     a.cut2 = cut * cut;
-    const size_t lds = (size_t)B.wcap * (3 * sizeof(double) + sizeof(int));
+    const size_t lds = (size_t)a.wcap * kStageBytes;
     hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.tile_rep.as<int>(), (const double4*)B.spos.as<double4>(), (const double4*)B.pos.as<double4>(),
                        (const int*)B.cell_of.as<int>(), (const int*)B.cell_start.as<int>(), (const int*)B.parts.as<int>(),
@@ -813,19 +829,21 @@ static void enq_nlist(Batch& B) {
 static void enq_refresh(Batch& B) {
     hipLaunchKernelGGL(k_b_refresh, dim3(B.n_gb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.gb_rep.as<int>(), (const int*)B.dyn.as<int>(), B.pos.as<double4>(),
-                       (const int*)B.g_owner.as<int>(), (const int*)B.g_code.as<int>());
+                       (const int*)B.g_owner.as<int>(), (const int*)B.g_code.as<int>(), (const int*)B.cell_inv.as<int>(),
+                       B.spos.as<double4>());
 }
 
-static void enq_force(Batch& B) {
+static void enq_force(Batch& B, bool kick) {
     const LJArgs a = batch_lj_args(B);
-    const size_t lds = (size_t)B.wcap * 3 * sizeof(double);
+    const size_t lds = (size_t)B.wcap * kStageBytes;
     const bool unit = B.eps == 1.0 && B.sigma == 1.0;
+    const double hdt = kick ? 0.5 * B.dt : 0.0;  // hdt = 0 leaves v unchanged (v + f * 0 == v)
 #define TMD_B_LJ(P, SH, U)                                                                                          \
     hipLaunchKernelGGL((k_b_lj_full<P, SH, U>), dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,                 \
                        (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
                        (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), \
                        (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), (const nb_t*)B.nb_list.as<nb_t>(), \
-                       a, B.force.as<double4>())
+                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt)
 #define TMD_B_LJ_U(P, SH) \
     do { if (unit) TMD_B_LJ(P, SH, 1); else TMD_B_LJ(P, SH, 0); } while (0)
 #define TMD_B_LJ_S(P) \
@@ -842,12 +860,8 @@ static void enq_force(Batch& B) {
 static void enq_int1(Batch& B) {
     hipLaunchKernelGGL(k_b_int1, dim3(B.n_eb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.eb_rep.as<int>(), B.pos.as<double4>(), B.vel.as<double4>(),
-                       (const double4*)B.force.as<double4>(), 0.5 * B.dt, B.dt);
-}
-
-static void enq_int2(Batch& B) {
-    hipLaunchKernelGGL(k_b_int2, dim3(B.n_eb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.eb_rep.as<int>(), B.vel.as<double4>(), (const double4*)B.force.as<double4>(), 0.5 * B.dt);
+                       (const double4*)B.force.as<double4>(), 0.5 * B.dt, B.dt, (const int*)B.cell_inv.as<int>(),
+                       B.spos.as<double4>());
 }
 
 // the main.cpp step loop over `steps` steps (rebuild when (j + 1) % reneigh == 0)
@@ -859,11 +873,9 @@ static void enq_steps(Batch& B, int steps) {
             enq_mirror(B);
             enq_nlist(B);
         } else {
-            enq_refresh(B);
-            enq_mirror(B);
+            enq_refresh(B);  // integration and refresh keep the mirror current
         }
-        enq_force(B);
-        enq_int2(B);
+        enq_force(B, true);
     }
     TMD_HIP(hipGetLastError());
 }
@@ -925,9 +937,8 @@ static bool setup_lists(Batch& B) {
             set_error(__FILE__, __LINE__, "batch: a tile neighbourhood exceeds the 16-bit window range; use a smaller tile");
             return false;
         }
-        const int per = 3 * (int)sizeof(double) + (int)sizeof(int);
         const int maxwin = fl[FL_MAXWIN];
-        B.wcap = std::min(maxwin + maxwin / 4 + 64, kLdsBudget / per);
+        B.wcap = std::min(maxwin + maxwin / 4 + 64, kLdsBudget / kStageBytes);
         enq_mirror(B);
         for (int k = 0; k < 4 && !has_error(); ++k) {
             reserve_lists(B);
@@ -939,7 +950,7 @@ static bool setup_lists(Batch& B) {
         }
         if (has_error()) return false;
         if (fl[FL_LIST] > 0) continue;
-        enq_force(B);
+        enq_force(B, false);
         TMD_HIP(hipStreamSynchronize(B.stream));
         B.ready = !has_error();
         return B.ready;
@@ -1055,7 +1066,7 @@ static bool run_steps(Batch& B, int steps, double* gpu_ms) {
 
 static void batch_thermo(Batch& B, double* out) {
     const LJArgs a = batch_lj_args(B);
-    const size_t lds = (size_t)B.wcap * (3 * sizeof(double) + sizeof(int));
+    const size_t lds = (size_t)B.wcap * kStageBytes;
     enq_mirror(B);
 #define TMD_B_EN(SH)                                                                                                 \
     hipLaunchKernelGGL(k_b_energy<SH>, dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,                          \

@@ -15,7 +15,7 @@ struct BinArgs {
 };
 
 __global__ void k_bin_count(const double4* __restrict__ pos, BinArgs a, int* __restrict__ cell_of, int* __restrict__ slot,
-                            int* __restrict__ count) {
+                            int* __restrict__ count, int* __restrict__ inv) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nall) return;
     const double4 p = pos[i];
@@ -31,6 +31,7 @@ __global__ void k_bin_count(const double4* __restrict__ pos, BinArgs a, int* __r
     }
     cell_of[i] = c;
     slot[i] = (c >= 0) ? atomicAdd(&count[c], 1) : -1;
+    inv[i] = -1;  // set by k_bin_sort for binned atoms
 }
 
 __global__ void k_bin_scatter(const int* __restrict__ cell_of, const int* __restrict__ slot, const int* __restrict__ start,
@@ -45,7 +46,7 @@ __global__ void k_bin_scatter(const int* __restrict__ cell_of, const int* __rest
 // number of locals in it (locals have the lowest indices, so they lead every
 // sorted segment).
 __global__ void k_bin_sort(const int* __restrict__ start, int ncells, int n, int* __restrict__ parts,
-                           int* __restrict__ local_count) {
+                           int* __restrict__ local_count, int* __restrict__ inv) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     const int b = start[c], e = start[c + 1];
@@ -61,6 +62,7 @@ __global__ void k_bin_sort(const int* __restrict__ start, int ncells, int n, int
     int nl = 0;
     while (b + nl < e && parts[b + nl] < n) ++nl;
     local_count[c] = nl;
+    for (int k = b; k < e; ++k) inv[parts[k]] = k;
 }
 
 // Tile work order (see tmd_window.hpp): one thread per tile counts the
@@ -145,17 +147,20 @@ void bin_particles(Sim& s) {
     s.work.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
     if (has_error()) return;
     BinArgs a{G.aabb[0], G.aabb[2], G.aabb[4], G.spacing, G.nx, G.ny, G.ncells, nall};
+    s.mirror_fresh = 0;
+    s.inv_valid = 0;
     const int nt = 256;
     TMD_HIP(hipMemsetAsync(s.cell_count.p, 0, sizeof(int) * (size_t)G.ncells, s.stream));
     if (nall > 0)
         hipLaunchKernelGGL(k_bin_count, dim3(div_up(nall, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(), a,
-                           s.cell_of.as<int>(), s.cell_slot.as<int>(), s.cell_count.as<int>());
+                           s.cell_of.as<int>(), s.cell_slot.as<int>(), s.cell_count.as<int>(), s.cell_inv.as<int>());
     exclusive_scan_int(s.cell_count.as<int>(), s.cell_start.as<int>(), G.ncells, s.cell_start.as<int>() + G.ncells, s);
     if (nall > 0)
         hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(nall, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_of.as<int>(),
                            (const int*)s.cell_slot.as<int>(), (const int*)s.cell_start.as<int>(), nall, s.cell_parts.as<int>());
     hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(), G.ncells,
-                       s.n, s.cell_parts.as<int>(), s.cell_count.as<int>());
+                       s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(), s.cell_inv.as<int>());
+    s.inv_valid = 1;
     // tile work order
     choose_tiles(s);
     const TileGeom tg = s.tiles;

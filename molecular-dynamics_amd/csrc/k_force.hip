@@ -39,30 +39,52 @@ namespace tmd {
         const int *__restrict__ parts, const int *__restrict__ work, const int *__restrict__ tile_start,           \
         const int *__restrict__ nbc, const nb_t *__restrict__ nbl
 
-template <int PREC, int SHIFT, int UNIT>
-__global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force) {
+template <int PREC, int SHIFT, int UNIT, int KICK>
+__global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
+                                                         double4* __restrict__ vel, double hdt) {
     extern __shared__ double dyn[];
+    double4* sp = reinterpret_cast<double4*>(dyn);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    const bool staged = stage_tile_aos(W, a.wcap, spos, dyn);
+    const bool staged = stage_tile_aos(W, a.wcap, spos, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
         const double4 pi = pos[i];
         const int cnt = nbc[w];
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
-        force[i] = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, dyn, spos, W, a)
-                          : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, dyn, spos, W, a);
+        const double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
+                                 : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
+        force[i] = f;
+        if (KICK) {
+            // velocity-Verlet final kick (k_final_integration's arithmetic)
+            double4 v = vel[i];
+            v.x = v.x + f.x * hdt;
+            v.y = v.y + f.y * hdt;
+            v.z = v.z + f.z * hdt;
+            vel[i] = v;
+        }
     }
 }
 
 __device__ __forceinline__ unsigned long long to_fix(double v) { return (unsigned long long)__double2ll_rn(v * kFixScale); }
 
+// Half-list LJ.  Each pair is evaluated once, by the lane of its higher
+// index (local-ghost pairs by the local side; no reaction on ghosts).  The
+// force on the lane's own atom is summed in registers in list order and
+// stored once (`own`).  Reactions -F on local neighbours are added, as
+// 64-bit fixed-point integers (scale 2^40), to LDS accumulators of the
+// staged window, and each window atom's accumulated reaction is added to
+// its global accumulator with one integer atomic per component when the
+// block ends.  Integer addition is associative, so the result does not
+// depend on the order in which lanes and blocks run.  A window too large
+// for LDS sends reactions straight to the global accumulators.
 template <int PREC, int SHIFT>
-__global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS, LJArgs a, unsigned long long* __restrict__ acc,
+__global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS, LJArgs a, int wcap_half,
+                                                         double4* __restrict__ own, unsigned long long* __restrict__ acc,
                                                          int* __restrict__ flag) {
     extern __shared__ double dyn[];
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -70,60 +92,131 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    double* sx = dyn;
-    double* sy = dyn + a.wcap;
-    double* sz = dyn + 2 * a.wcap;
-    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
-    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
-    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
-    const int w = w0 + at;
-    const int i = work[w];
-    const double4 pi = pos[i];
-    const int cnt = nbc[w];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    bool big = false;
-    for (int k = 0; k < cnt; ++k) {
-        const int t = nbl[nb16_slot(w, k, a.cap)];
-        int j;
-        double xj, yj, zj;
-        if (staged) {
-            j = sj[t];
-            xj = sx[t]; yj = sy[t]; zj = sz[t];
-        } else {
-            const int kk = win_index(t, W);
-            j = parts[kk];
-            const double4 p = spos[kk];
-            xj = p.x; yj = p.y; zj = p.z;
-        }
-        const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        if (rsq < a.rc2) {
-            double Fx, Fy, Fz;
-            if (PREC == PREC_FP64) {
-                const double f = lj_f64<SHIFT>(a, rsq);
-                Fx = dx * f; Fy = dy * f; Fz = dz * f;
-            } else {
-                const float f = lj_f32<SHIFT>(a, (float)rsq);
-                Fx = (double)((float)dx * f); Fy = (double)((float)dy * f); Fz = (double)((float)dz * f);
-            }
-            fx = fx + Fx; fy = fy + Fy; fz = fz + Fz;
-            if (j < a.n) {
-                big |= fabs(Fx) > kFixMax || fabs(Fy) > kFixMax || fabs(Fz) > kFixMax;
-                atomicAdd(&acc[3 * (size_t)j + 0], to_fix(-Fx));
-                atomicAdd(&acc[3 * (size_t)j + 1], to_fix(-Fy));
-                atomicAdd(&acc[3 * (size_t)j + 2], to_fix(-Fz));
-            }
-        }
+    double4* sp = reinterpret_cast<double4*>(dyn);
+    unsigned long long* racc = reinterpret_cast<unsigned long long*>(sp + wcap_half);
+    const bool staged = stage_tile_xyzj(W, wcap_half, spos, parts, sp);
+    const int S = win_size(W);
+    if (staged) {
+        for (int t = threadIdx.x; t < 3 * S; t += blockDim.x) racc[t] = 0ull;
+        __syncthreads();
     }
-    big |= fabs(fx) > kFixMax || fabs(fy) > kFixMax || fabs(fz) > kFixMax;
-    atomicAdd(&acc[3 * (size_t)i + 0], to_fix(fx));
-    atomicAdd(&acc[3 * (size_t)i + 1], to_fix(fy));
-    atomicAdd(&acc[3 * (size_t)i + 2], to_fix(fz));
+    bool big = false;
+    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+        const int w = w0 + at;
+        const int i = work[w];
+        const double4 pi = pos[i];
+        const int cnt = nbc[w];
+        const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const int ng = (cnt + 7) >> 3;
+        int4 q = ng > 0 ? lp[0] : make_int4(0, 0, 0, 0);
+        for (int gi = 0; gi < ng; ++gi) {
+            const int4 qn = (gi + 1 < ng) ? lp[(size_t)(gi + 1) * 64] : q;
+            const Nb8 e = unpack8(q);
+            q = qn;
+            const int k0 = gi * 8;
+            double dx[8], dy[8], dz[8], rs[8];
+            int jj[8], tt[8];
+            bool ok[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool valid = k0 + u < cnt;
+                const int t = valid ? e.t[u] : 0;
+                double xj, yj, zj;
+                if (staged) {
+                    const double4 r = sp[t];
+                    xj = r.x; yj = r.y; zj = r.z;
+                    jj[u] = (int)__double_as_longlong(r.w);
+                } else {
+                    const int kk = win_index(t, W);
+                    const double4 r = spos[kk];
+                    xj = r.x; yj = r.y; zj = r.z;
+                    jj[u] = parts[kk];
+                }
+                tt[u] = t;
+                dx[u] = pi.x - xj;
+                dy[u] = pi.y - yj;
+                dz[u] = pi.z - zj;
+                const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+                ok[u] = valid && rsq < a.rc2;
+                rs[u] = ok[u] ? rsq : 1.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                double fm;
+                if (PREC == PREC_FP64) {
+                    const double f = lj_f64u<SHIFT, 0>(a, rs[u]);
+                    fm = ok[u] ? f : 0.0;
+                } else {
+                    const float f = lj_f32u<SHIFT, 0>(a, (float)rs[u]);
+                    fm = (double)(ok[u] ? f : 0.0f);
+                }
+                const double Fx = dx[u] * fm, Fy = dy[u] * fm, Fz = dz[u] * fm;
+                fx = fx + Fx;
+                fy = fy + Fy;
+                fz = fz + Fz;
+                if (ok[u] && jj[u] < a.n) {
+                    big |= fabs(Fx) > kFixMax || fabs(Fy) > kFixMax || fabs(Fz) > kFixMax;
+                    const unsigned long long rx = to_fix(-Fx), ry = to_fix(-Fy), rz = to_fix(-Fz);
+                    if (staged) {
+                        atomicAdd(&racc[3 * tt[u] + 0], rx);
+                        atomicAdd(&racc[3 * tt[u] + 1], ry);
+                        atomicAdd(&racc[3 * tt[u] + 2], rz);
+                    } else {
+                        atomicAdd(&acc[3 * (size_t)jj[u] + 0], rx);
+                        atomicAdd(&acc[3 * (size_t)jj[u] + 1], ry);
+                        atomicAdd(&acc[3 * (size_t)jj[u] + 2], rz);
+                    }
+                }
+            }
+        }
+        own[i] = make_double4(fx, fy, fz, 0.0);
+    }
     if (big) atomicOr(flag, 1);
+    if (staged) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < S; t += blockDim.x) {
+            const int j = (int)__double_as_longlong(sp[t].w);
+            if (j < a.n) {
+                const unsigned long long rx = racc[3 * t + 0], ry = racc[3 * t + 1], rz = racc[3 * t + 2];
+                if ((rx | ry | rz) != 0ull) {
+                    atomicAdd(&acc[3 * (size_t)j + 0], rx);
+                    atomicAdd(&acc[3 * (size_t)j + 1], ry);
+                    atomicAdd(&acc[3 * (size_t)j + 2], rz);
+                }
+            }
+        }
     }
 }
 
-// fixed point -> FP64 forces; clears the accumulators for the next step
+// force = own sum + accumulated reactions; clears the accumulators; with
+// KICK, the velocity-Verlet final kick v += F dt/2
+template <int KICK>
+__global__ void k_half_finish(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force,
+                              double4* __restrict__ vel, double hdt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long ax = (long long)acc[3 * (size_t)i + 0];
+    const long long ay = (long long)acc[3 * (size_t)i + 1];
+    const long long az = (long long)acc[3 * (size_t)i + 2];
+    double4 f = force[i];
+    f.x = f.x + (double)ax * kFixInv;
+    f.y = f.y + (double)ay * kFixInv;
+    f.z = f.z + (double)az * kFixInv;
+    force[i] = f;
+    acc[3 * (size_t)i + 0] = 0ull;
+    acc[3 * (size_t)i + 1] = 0ull;
+    acc[3 * (size_t)i + 2] = 0ull;
+    if (KICK) {
+        double4 v = vel[i];
+        v.x = v.x + f.x * hdt;
+        v.y = v.y + f.y * hdt;
+        v.z = v.z + f.z * hdt;
+        vel[i] = v;
+    }
+}
+
+// fixed-point accumulators -> force (DEM half lists)
 __global__ void k_fix_to_force(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -316,11 +409,9 @@ static void check_fix_flag(Sim& s) {
         (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), (const int*)s.tile_start.as<int>(),     \
         (const int*)s.nb_count.as<int>(), (const nb_t*)s.nb_list.as<nb_t>()
 
-static size_t lds_bytes(const Sim& s, bool with_index) {
-    return (size_t)window_cap(s, true) * (3 * sizeof(double) + (with_index ? sizeof(int) : 0));
-}
+static size_t lds_bytes(const Sim& s, bool) { return (size_t)window_cap(s) * kStageBytes; }
 
-void compute_lj(Sim& s, int half, double rc, double eps, double sigma) {
+void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt) {
     if (has_error()) return;
     TMD_REQUIRE(s.nb_valid, "compute forces: no neighbour list (call md_assemble_neighborlists first)");
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
@@ -330,15 +421,22 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma) {
     const int shift = s.lj_shift;
     hipEvent_t ev = nullptr;
     s.timer.begin(K_MIRROR, s.stream, &ev);
-    refresh_mirror(s);
+    if (!s.mirror_fresh) refresh_mirror(s);
     s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
     if (!half) {
         double4* F = s.force.as<double4>();
         const size_t lds = lds_bytes(s, false);
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
-#define TMD_LJ_FULL(P, SH, U) \
-    hipLaunchKernelGGL((k_lj_full<P, SH, U>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, F)
+        const double hdt = 0.5 * kick_dt;
+        double4* V = s.vel.as<double4>();
+#define TMD_LJ_FULL(P, SH, U)                                                                                          \
+    do {                                                                                                               \
+        if (kick_dt > 0.0)                                                                                             \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
+        else                                                                                                           \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
+    } while (0)
         if (s.precision == PREC_FP64) {
             if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }
             else { if (unit) TMD_LJ_FULL(PREC_FP64, 0, 1); else TMD_LJ_FULL(PREC_FP64, 0, 0); }
@@ -355,14 +453,25 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma) {
         int* flag = s.counters.as<int>() + 4;
         TMD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s.stream));
         unsigned long long* A = s.acc.as<unsigned long long>();
+        // window records (32 B) + three 64-bit reaction accumulators per atom
+        const int per = kStageBytes + 3 * (int)sizeof(unsigned long long);
+        const int wh = std::min(s.max_window, std::min(s.lds_limit, kLdsHalfBudget) / per);
+        const size_t lds = (size_t)wh * per;
+        double4* F = s.force.as<double4>();
+#define TMD_LJ_HALF(P, SH) \
+    hipLaunchKernelGGL((k_lj_half<P, SH>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, wh, F, A, flag)
         if (s.precision == PREC_FP64) {
-            if (shift) hipLaunchKernelGGL((k_lj_half<PREC_FP64, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
-            else hipLaunchKernelGGL((k_lj_half<PREC_FP64, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
+            if (shift) TMD_LJ_HALF(PREC_FP64, 1); else TMD_LJ_HALF(PREC_FP64, 0);
         } else {
-            if (shift) hipLaunchKernelGGL((k_lj_half<PREC_FP32_MIXED, 1>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
-            else hipLaunchKernelGGL((k_lj_half<PREC_FP32_MIXED, 0>), dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, A, flag);
+            if (shift) TMD_LJ_HALF(PREC_FP32_MIXED, 1); else TMD_LJ_HALF(PREC_FP32_MIXED, 0);
         }
-        hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, s.force.as<double4>());
+#undef TMD_LJ_HALF
+        if (kick_dt > 0.0)
+            hipLaunchKernelGGL(k_half_finish<1>, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F, s.vel.as<double4>(),
+                               0.5 * kick_dt);
+        else
+            hipLaunchKernelGGL(k_half_finish<0>, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F, s.vel.as<double4>(),
+                               0.0);
     }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
@@ -375,7 +484,7 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
     if (has_error() || s.n == 0) return;
     const double radius = s.dem_radius > 0.0 ? s.dem_radius : 0.5 * rc;
-    DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n, window_cap(s, true), s.tiles};
+    DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n, window_cap(s), s.tiles};
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     ensure_acc(s);
     int* flag = s.counters.as<int>() + 4;
@@ -386,7 +495,7 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     const size_t lds = lds_bytes(s, true);
     hipEvent_t ev = nullptr;
     s.timer.begin(K_MIRROR, s.stream, &ev);
-    refresh_mirror(s);
+    if (!s.mirror_fresh) refresh_mirror(s);
     s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
     if (s.precision == PREC_FP64) {
@@ -405,7 +514,7 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
 void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part) {
     LJArgs a = make_lj_args(s, rc, eps, sigma);
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
-    refresh_mirror(s);
+    if (!s.mirror_fresh) refresh_mirror(s);
     if (s.lj_shift)
         hipLaunchKernelGGL(k_lj_energy<1>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
     else

@@ -122,7 +122,8 @@ __global__ void __launch_bounds__(kScanThreads) k_border_scatter(double4* __rest
 // FRESH refresh: ghost = owner + image * L (bit-identical to the rebuild's
 // chained construction because each coordinate receives exactly one shift).
 __global__ void k_ghost_refresh(double4* __restrict__ pos, double4* __restrict__ vel, const int* __restrict__ g_owner,
-                                const int* __restrict__ g_code, int n, int ng, double L0, double L1, double L2, int with_vel) {
+                                const int* __restrict__ g_code, int n, int ng, double L0, double L1, double L2, int with_vel,
+                                const int* __restrict__ inv, double4* __restrict__ spos) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int o = g_owner[g];
@@ -136,6 +137,10 @@ __global__ void k_ghost_refresh(double4* __restrict__ pos, double4* __restrict__
     q.w = p.w;
     pos[n + g] = q;
     if (with_vel) vel[n + g] = vel[o];
+    if (inv) {
+        const int k = inv[n + g];
+        if (k >= 0) spos[k] = q;
+    }
 }
 
 // REF_LAG refresh (diagnostic): the reference's pack from the immediate
@@ -156,11 +161,16 @@ __global__ void k_ghost_pack_lag(const double4* __restrict__ pos, const int* __r
 }
 
 __global__ void k_ghost_unpack_lag(double4* __restrict__ pos, double4* __restrict__ vel, const int* __restrict__ g_owner,
-                                   const double4* __restrict__ buf, int n, int ng, int with_vel) {
+                                   const double4* __restrict__ buf, int n, int ng, int with_vel, const int* __restrict__ inv,
+                                   double4* __restrict__ spos) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     pos[n + g] = buf[g];
     if (with_vel) vel[n + g] = vel[g_owner[g]];
+    if (inv) {
+        const int k = inv[n + g];
+        if (k >= 0) spos[k] = buf[g];
+    }
 }
 
 static void ensure_atom_capacity(Sim& s, int need);
@@ -168,6 +178,8 @@ static void ensure_atom_capacity(Sim& s, int need);
 void build_ghosts(Sim& s) {
     if (has_error()) return;
     s.ng = 0;
+    s.inv_valid = 0;
+    s.mirror_fresh = 0;
     const Geom& G = s.g;
     const double dspacing = G.spacing * 2.0;
     int* counters = s.counters.as<int>();
@@ -215,11 +227,13 @@ void refresh_ghosts(Sim& s) {
                            (const int*)s.g_src.as<int>(), (const int*)s.g_code.as<int>(), s.ng, s.g.len[0], s.g.len[1], s.g.len[2],
                            s.gbuf.as<double4>());
         hipLaunchKernelGGL(k_ghost_unpack_lag, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.vel.as<double4>(),
-                           (const int*)s.g_owner.as<int>(), (const double4*)s.gbuf.as<double4>(), s.n, s.ng, with_vel);
+                           (const int*)s.g_owner.as<int>(), (const double4*)s.gbuf.as<double4>(), s.n, s.ng, with_vel,
+                           s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr, s.spos.as<double4>());
     } else {
         hipLaunchKernelGGL(k_ghost_refresh, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.vel.as<double4>(),
                            (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0], s.g.len[1],
-                           s.g.len[2], with_vel);
+                           s.g.len[2], with_vel, s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr,
+                           s.spos.as<double4>());
     }
     TMD_HIP(hipGetLastError());
 }
@@ -238,6 +252,8 @@ static void ensure_atom_capacity(Sim& s, int need) {
     s.cell_of.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.cell_slot.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.cell_parts.reserve(sizeof(int) * (size_t)cap, false, s.stream);
+    s.cell_inv.reserve(sizeof(int) * (size_t)cap, false, s.stream);
+    s.inv_valid = 0;
     s.cap_all = cap;
 }
 

@@ -6,9 +6,11 @@
 
 namespace tmd {
 
-// v += f * (dt/2); x += v * dt
+// v += f * (dt/2); x += v * dt; with MIRROR the cell-ordered position
+// mirror is kept current too (spos[inv[i]] = x)
+template <int MIRROR>
 __global__ void k_initial_integration(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ force,
-                                      int n, double hdt, double dt) {
+                                      int n, double hdt, double dt, const int* __restrict__ inv, double4* __restrict__ spos) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 v = vel[i];
@@ -22,6 +24,10 @@ __global__ void k_initial_integration(double4* __restrict__ pos, double4* __rest
     x.z = x.z + v.z * dt;
     vel[i] = v;
     pos[i] = x;
+    if (MIRROR) {
+        const int k = inv[i];
+        if (k >= 0) spos[k] = x;
+    }
 }
 
 __global__ void k_final_integration(double4* __restrict__ vel, const double4* __restrict__ force, int n, double hdt) {
@@ -67,8 +73,15 @@ void initial_integration(Sim& s, double dt) {
     const int nt = 256;
     hipEvent_t ev = nullptr;
     s.timer.begin(K_INTEGRATE_1, s.stream, &ev);
-    hipLaunchKernelGGL(k_initial_integration, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.vel.as<double4>(),
-                       (const double4*)s.force.as<double4>(), s.n, 0.5 * dt, dt);
+    if (s.inv_valid)
+        hipLaunchKernelGGL(k_initial_integration<1>, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
+                           s.vel.as<double4>(), (const double4*)s.force.as<double4>(), s.n, 0.5 * dt, dt,
+                           (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+    else
+        hipLaunchKernelGGL(k_initial_integration<0>, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
+                           s.vel.as<double4>(), (const double4*)s.force.as<double4>(), s.n, 0.5 * dt, dt, (const int*)nullptr,
+                           (double4*)nullptr);
+    if (!s.inv_valid) s.mirror_fresh = 0;
     s.timer.end(K_INTEGRATE_1, s.stream, ev);
     TMD_HIP(hipGetLastError());
 }
@@ -93,6 +106,7 @@ void enforce_pbc(Sim& s) {
         a.len[d] = s.g.len[d];
     }
     const int nt = 256;
+    s.mirror_fresh = 0;
     hipLaunchKernelGGL(k_enforce_pbc, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.n, a);
     TMD_HIP(hipGetLastError());
 }

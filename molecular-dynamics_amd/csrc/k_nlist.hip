@@ -35,25 +35,19 @@ __global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restric
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    double* sx = dyn;
-    double* sy = dyn + a.wcap;
-    double* sz = dyn + 2 * a.wcap;
-    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
-    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    double4* sp = reinterpret_cast<double4*>(dyn);
+    const bool staged = stage_tile_xyzj(W, a.wcap, spos, parts, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], start, a, W, staged, sx, sy, sz, sj, spos, parts, nb_list);
+        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], start, a, W, staged, sp, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(overflow, cnt);
     }
 }
 
 // ---------------------------------------------------------------------------
-int window_cap(const Sim& s, bool with_index) {
-    const int per = 3 * (int)sizeof(double) + (with_index ? (int)sizeof(int) : 0);
-    return std::min(s.max_window, kLdsBudget / per);
-}
+int window_cap(const Sim& s) { return std::min(s.max_window, kLdsBudget / kStageBytes); }
 
 void build_windows(Sim& s) {
     if (has_error()) return;
@@ -101,6 +95,7 @@ void refresh_mirror(Sim& s) {
                        (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
                        s.spos.as<double4>());
     TMD_HIP(hipGetLastError());
+    s.mirror_fresh = 1;
 }
 
 void build_neighbor_lists(Sim& s, int half, double cutoff) {
@@ -119,9 +114,9 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         s.nb_list.reserve(sizeof(nb_t) * (size_t)ntile64 * 64 * (size_t)s.nb_cap, false, s.stream);
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
-        const int wcap = window_cap(s, true);
+        const int wcap = window_cap(s);
         NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
-        const size_t lds = (size_t)wcap * (3 * sizeof(double) + sizeof(int));
+        const size_t lds = (size_t)wcap * kStageBytes;
         if (s.n > 0)
             hipLaunchKernelGGL(k_nlist, dim3(ntiles), dim3(kTileThreads), lds, s.stream, (const double4*)s.spos.as<double4>(),
                                (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(),

@@ -139,6 +139,8 @@ static void alloc_atoms(Sim& s, int cap) {
     s.cell_of.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.cell_slot.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.cell_parts.reserve(sizeof(int) * (size_t)cap, false, s.stream);
+    s.cell_inv.reserve(sizeof(int) * (size_t)cap, false, s.stream);
+    s.inv_valid = 0;
     s.force.reserve(sizeof(double4) * (size_t)std::max(s.n, 1), false, s.stream);
     if (cap > s.cap_all) s.cap_all = cap;
 }
@@ -157,6 +159,7 @@ static void upload(Sim& s) {
         p[i] = make_double4(s.h_pos[3 * i], s.h_pos[3 * i + 1], s.h_pos[3 * i + 2], 0.0);
         v[i] = make_double4(s.h_vel[3 * i], s.h_vel[3 * i + 1], s.h_vel[3 * i + 2], 0.0);
     }
+    s.mirror_fresh = 0;
     TMD_HIP(hipMemcpyAsync(s.pos.p, p.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.vel.p, v.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.mass.p, s.h_mass.data(), sizeof(double) * nall, hipMemcpyHostToDevice, s.stream));
@@ -422,7 +425,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.work_start, &s.work, &s.tile_start, &s.spos,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.tile_start, &s.spos,
                       &s.nb_count, &s.nb_list,
                       &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
@@ -432,6 +435,8 @@ void md_deallocate_grid() {
     s.cap_all = 0;
     s.nb_valid = 0;
     s.windows_valid = 0;
+    s.inv_valid = 0;
+    s.mirror_fresh = 0;
     s.nbinned = 0;
     s.dev_valid = false;
 }
@@ -752,9 +757,12 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
             refresh_ghosts(s);
             s.timer.end(K_GHOST_SYNC, s.stream, ev);
         }
-        if (dem) compute_dem(s, hf, cutoff, p0, p1, p2, p3);
-        else compute_lj(s, hf, cutoff, p0, p1);
-        final_integration(s, dt);
+        if (dem) {
+            compute_dem(s, hf, cutoff, p0, p1, p2, p3);
+            final_integration(s, dt);
+        } else {
+            compute_lj(s, hf, cutoff, p0, p1, dt);  // final kick fused
+        }
         ++done;
     }
     s.sync_calls = saved_sync;

@@ -96,6 +96,8 @@ struct Sim {
     TileGeom tiles{};   // tile decomposition of the cell grid
     int tile_req[3] = {0, 0, 0};  // requested cells per tile (0: automatic)
     int windows_valid = 0;
+    int inv_valid = 0;      // cell_inv matches the current atom set (binning after the last ghost build)
+    int mirror_fresh = 0;   // spos[k] == pos[cell_parts[k]] for every binned atom
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
@@ -115,6 +117,7 @@ struct Sim {
     DevBuf pos, vel, force, mass;      // double4 x3, double
     DevBuf g_owner, g_src, g_code;     // ghost table
     DevBuf cell_of, cell_slot, cell_count, cell_start, cell_parts;
+    DevBuf cell_inv;                   // CSR index of each atom (-1: outside the grid)
     DevBuf work_start, work;           // per-tile local counts; locals in tile work order (work slots)
     DevBuf tile_start;                 // first work slot of each tile [ntiles + 1]
     DevBuf spos;                       // cell-ordered position mirror (double4)
@@ -164,12 +167,14 @@ TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz);
 
 // tile windows (after binning), position mirror, neighbour lists
 void build_windows(Sim& s);
-int window_cap(const Sim& s, bool with_index);  // staged atoms per block (LDS budget)
+int window_cap(const Sim& s);  // staged window atoms per block (LDS budget, kStageBytes each)
 void refresh_mirror(Sim& s);
 void build_neighbor_lists(Sim& s, int half, double cutoff);
 
 // forces
-void compute_lj(Sim& s, int half, double rc, double eps, double sigma);
+// kick_dt > 0 (full lists): the final velocity-Verlet kick v += F dt/2 is
+// fused into the force kernel
+void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt = 0.0);
 void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu);
 void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs);
 

@@ -41,7 +41,7 @@ inline LJArgs make_lj_args(const Sim& s, double rc, double eps, double sigma) {
     a.cap = s.nb_cap;
     a.nwork = s.n;
     a.n = s.n;
-    a.wcap = window_cap(s, true);
+    a.wcap = window_cap(s);
     a.g = s.tiles;
     return a;
 }

@@ -72,18 +72,12 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 // Neighbour position of window offset t: LDS (x, y, z interleaved) when the
 // window is staged, else the cell-ordered mirror.
 template <bool STAGED>
-__device__ __forceinline__ void win_pos(int t, const double* __restrict__ sp, const double4* __restrict__ spos, const TileWin& W,
+__device__ __forceinline__ void win_pos(int t, const double4* __restrict__ sp, const double4* __restrict__ spos, const TileWin& W,
                                         double& x, double& y, double& z) {
-    if (STAGED) {
-        x = sp[3 * t + 0];
-        y = sp[3 * t + 1];
-        z = sp[3 * t + 2];
-    } else {
-        const double4 p = spos[win_index(t, W)];
-        x = p.x;
-        y = p.y;
-        z = p.z;
-    }
+    const double4 p = STAGED ? sp[t] : spos[win_index(t, W)];
+    x = p.x;
+    y = p.y;
+    z = p.z;
 }
 
 // Full-list force on one atom (lane).  Entries are read 8 at a time; the 8
@@ -93,13 +87,17 @@ __device__ __forceinline__ void win_pos(int t, const double* __restrict__ sp, co
 // sum starts at +0.0 and never becomes -0.0), so the FP64 result equals the
 // sequential reference sum bit for bit.
 template <int PREC, int SHIFT, int UNIT, bool STAGED>
-__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const double* __restrict__ sp,
+__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const double4* __restrict__ sp,
                                                 const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
     const int ng = (cnt + 7) >> 3;
+    // the next group of 8 entries is loaded while the current one is used
+    int4 q = ng > 0 ? lp[0] : make_int4(0, 0, 0, 0);
     for (int gi = 0; gi < ng; ++gi) {
-        const Nb8 e = unpack8(lp[(size_t)gi * 64]);
+        const int4 qn = (gi + 1 < ng) ? lp[(size_t)(gi + 1) * 64] : q;
+        const Nb8 e = unpack8(q);
+        q = qn;
         const int k0 = gi * 8;
         double dx[8], dy[8], dz[8], rs[8];
         bool ok[8];
@@ -202,14 +200,20 @@ struct NbArgs {
 };
 
 // Verlet list of local atom i (work slot w, cell c of the tile's grid) from
-// the staged window: 9 stencil rows of 3 cells in the reference's order,
-// `rsq <= cut2` in FP64 without contraction, u16 window offsets.  Returns
-// the count (entries beyond the capacity are counted, not stored).
+// the staged window (x, y, z, j records): 9 stencil rows of 3 cells in the
+// reference's order, `rsq <= cut2` in FP64 without contraction.  Accepted
+// window offsets are packed 8 to a 16-byte group in registers and each full
+// group is written with one store (one list column per lane, so a wave's
+// stores fall in one 1 KiB stripe).  Returns the count; groups beyond the
+// capacity are counted, not stored.
 __device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi, const int* __restrict__ start,
-                                          const NbArgs& a, const TileWin& W, bool staged, const double* sx, const double* sy,
-                                          const double* sz, const int* sj, const double4* __restrict__ spos,
-                                          const int* __restrict__ parts, nb_t* __restrict__ nb_list) {
+                                          const NbArgs& a, const TileWin& W, bool staged, const double4* sp,
+                                          const double4* __restrict__ spos, const int* __restrict__ parts,
+                                          nb_t* __restrict__ nb_list) {
     const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
+    int4* const col = reinterpret_cast<int4*>(nb_list) + (size_t)(w >> 6) * (size_t)a.cap * 8 + (w & 63);
+    const int ngroups = a.cap >> 3;
+    unsigned long long lo = 0ull, hi = 0ull;
     int cnt = 0;
     for (int r = 0; r < 9; ++r) {
         int t0, t1;
@@ -224,8 +228,9 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi,
                 int j;
                 double xj, yj, zj;
                 if (staged) {
-                    j = sj[t];
-                    xj = sx[t]; yj = sy[t]; zj = sz[t];
+                    const double4 q = sp[t];
+                    xj = q.x; yj = q.y; zj = q.z;
+                    j = (int)__double_as_longlong(q.w);
                 } else {
                     const int kk = win_index(t, W);
                     j = parts[kk];
@@ -240,11 +245,26 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi,
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                if (ok[u] && cnt < a.cap) nb_list[nb16_slot(w, cnt, a.cap)] = (nb_t)(tb + u);
-                cnt += ok[u] ? 1 : 0;
+                if (ok[u]) {
+                    const int p = cnt & 7;
+                    const unsigned long long v = (unsigned long long)(unsigned)(tb + u) << (16 * (p & 3));
+                    if (p < 4) lo |= v;
+                    else hi |= v;
+                    ++cnt;
+                    if ((cnt & 7) == 0) {
+                        const int grp = (cnt >> 3) - 1;
+                        if (grp < ngroups)
+                            col[(size_t)grp * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
+                                                              (int)(unsigned)(hi >> 32));
+                        lo = hi = 0ull;
+                    }
+                }
             }
         }
     }
+    if ((cnt & 7) != 0 && (cnt >> 3) < ngroups)
+        col[(size_t)(cnt >> 3) * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
+                                                 (int)(unsigned)(hi >> 32));
     return cnt;
 }
 

@@ -27,6 +27,8 @@ constexpr int kMaxTile = 8;        // cells per tile dimension, at most
 constexpr int kMaxSeg = (kMaxTile + 2) * (kMaxTile + 2);
 constexpr int kWinMax = 65535;     // window offsets are 16-bit
 constexpr int kLdsBudget = 80 * 1024;  // LDS per block: two blocks per 160 KiB CU
+constexpr int kStageBytes = 32;       // LDS bytes per staged window atom (a double4 record)
+constexpr int kLdsHalfBudget = 144 * 1024;  // half-list LJ: window records + reaction accumulators (one block per CU)
 
 typedef unsigned short nb_t;
 
@@ -129,18 +131,32 @@ __device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const dou
     return staged;
 }
 
-// Stage the window's FP64 positions interleaved (x, y, z per atom) into LDS
-// if it fits wcap atoms.  Whole block.  Returns whether it was staged.
-__device__ __forceinline__ bool stage_tile_aos(const TileWin& W, int wcap, const double4* __restrict__ spos, double* sp) {
+// Stage the window as one 32-byte record per atom, (x, y, z, atom index as
+// the bit pattern of the fourth double), read back with two 16-byte LDS
+// loads.  Whole block.  Returns whether it was staged (S <= wcap).
+__device__ __forceinline__ bool stage_tile_xyzj(const TileWin& W, int wcap, const double4* __restrict__ spos,
+                                                const int* __restrict__ parts, double4* sp) {
     const int S = win_size(W);
     const bool staged = S <= wcap;
     if (staged) {
         for (int t = threadIdx.x; t < S; t += blockDim.x) {
-            const double4 p = spos[win_index(t, W)];
-            sp[3 * t + 0] = p.x;
-            sp[3 * t + 1] = p.y;
-            sp[3 * t + 2] = p.z;
+            const int k = win_index(t, W);
+            const double4 p = spos[k];
+            sp[t] = make_double4(p.x, p.y, p.z, __longlong_as_double((long long)parts[k]));
         }
+    }
+    __syncthreads();
+    return staged;
+}
+
+// Stage the window's FP64 positions as double4 records into LDS (two
+// 16-byte loads per neighbour) if it fits wcap atoms.  Whole block.
+// Returns whether it was staged.
+__device__ __forceinline__ bool stage_tile_aos(const TileWin& W, int wcap, const double4* __restrict__ spos, double4* sp) {
+    const int S = win_size(W);
+    const bool staged = S <= wcap;
+    if (staged) {
+        for (int t = threadIdx.x; t < S; t += blockDim.x) sp[t] = spos[win_index(t, W)];
     }
     __syncthreads();
     return staged;
