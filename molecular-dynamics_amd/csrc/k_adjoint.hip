@@ -143,8 +143,8 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_adjoint(const double4* __re
         xb.z += az;
         xbar[i] = xb;
     }
-    const double be = block_sum_double<kTileThreads>(g_eps);
-    const double bs = block_sum_double<kTileThreads>(g_sig);
+    const double be = block_sum_double_any(g_eps);
+    const double bs = block_sum_double_any(g_sig);
     if (threadIdx.x == 0) {
         partials[2 * tile + 0] = be;
         partials[2 * tile + 1] = bs;
@@ -237,7 +237,7 @@ static void launch_lj_adjoint(Sim& s, double rc, double eps, double sigma, const
     const size_t lds = (size_t)a.wcap * kStageBytes;
     double* part = partials(s, 2 * (size_t)ntiles);
 #define TMD_ADJ(P, SH)                                                                                                  \
-    hipLaunchKernelGGL((k_lj_adjoint<P, SH>), dim3(ntiles), dim3(kTileThreads), lds, s.stream,                          \
+    hipLaunchKernelGGL((k_lj_adjoint<P, SH>), dim3(ntiles), dim3(s.tile_threads), lds, s.stream,                          \
                        (const double4*)s.pos.as<double4>(), (const double4*)s.spos.as<double4>(),                       \
                        (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), \
                        (const int*)s.tile_start.as<int>(), (const int*)s.nb_count.as<int>(),                            \

@@ -65,24 +65,6 @@ __device__ __forceinline__ void b_img_decode(int c, int& ix, int& iy, int& iz) {
 }
 __device__ __forceinline__ double b_comp(const double4& p, int d) { return d == 0 ? p.x : (d == 1 ? p.y : p.z); }
 
-// Deterministic block sum of doubles for any blockDim.x multiple of 64
-// (fixed shuffle tree, fixed wave order).  Valid in thread 0.
-__device__ __forceinline__ double block_sum_double_any(double v) {
-    __shared__ double wsum[kTileThreads / kWave];
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_down(v, d, kWave);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
-    if (lane == 0) wsum[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        const int nw = (int)blockDim.x / kWave;
-        for (int w = 0; w < nw; ++w) r += wsum[w];
-    }
-    __syncthreads();
-    return r;
-}
 
 __global__ void k_b_fill(int* __restrict__ p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,8 +422,8 @@ __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restr
     const int* st = start + D.c0;
     __shared__ TileWin W;
     tile_window(tile - D.t0, st, D.g, W);
-    double4* sp = reinterpret_cast<double4*>(dyn);
-    const bool staged = stage_tile_xyzj(W, a.wcap, spos, parts, sp);
+    const WinStage sp = win_stage(dyn, a.wcap);
+    const bool staged = stage_window(W, a.wcap, spos, parts, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
@@ -459,14 +441,14 @@ __global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __res
                                                             const nb_t* __restrict__ nbl, LJArgs a, double4* __restrict__ force,
                                                             double4* __restrict__ vel, double hdt) {
     extern __shared__ double dyn[];
-    double4* sp = reinterpret_cast<double4*>(dyn);
+    const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
     const RepDesc& D = R[tile_rep[tile]];
     __shared__ TileWin W;
     tile_window(tile - D.t0, start + D.c0, D.g, W);
-    const bool staged = stage_tile_aos(W, a.wcap, spos, sp);
+    const bool staged = stage_window(W, a.wcap, spos, nullptr, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];

@@ -43,13 +43,13 @@ template <int PREC, int SHIFT, int UNIT, int KICK>
 __global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
                                                          double4* __restrict__ vel, double hdt) {
     extern __shared__ double dyn[];
-    double4* sp = reinterpret_cast<double4*>(dyn);
+    const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    const bool staged = stage_tile_aos(W, a.wcap, spos, sp);
+    const bool staged = stage_window(W, a.wcap, spos, nullptr, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
@@ -92,9 +92,10 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    double4* sp = reinterpret_cast<double4*>(dyn);
-    unsigned long long* racc = reinterpret_cast<unsigned long long*>(sp + wcap_half);
-    const bool staged = stage_tile_xyzj(W, wcap_half, spos, parts, sp);
+    const WinStage sp = win_stage(dyn, wcap_half);
+    // wcap_half is even, so the accumulators start 8-byte aligned
+    unsigned long long* racc = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(dyn) + (size_t)kStageBytes * wcap_half);
+    const bool staged = stage_window(W, wcap_half, spos, parts, sp);
     const int S = win_size(W);
     if (staged) {
         for (int t = threadIdx.x; t < 3 * S; t += blockDim.x) racc[t] = 0ull;
@@ -109,11 +110,10 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const int ng = (cnt + 7) >> 3;
-        int4 q = ng > 0 ? lp[0] : make_int4(0, 0, 0, 0);
+        int4 q = lp[0];  // group 0 of the column always exists (capacity >= 8)
         for (int gi = 0; gi < ng; ++gi) {
-            const int4 qn = (gi + 1 < ng) ? lp[(size_t)(gi + 1) * 64] : q;
-            const Nb8 e = unpack8(q);
-            q = qn;
+            const int4 e = q;
+            q = lp[(size_t)min(gi + 1, ng - 1) * 64];  // next group (clamped: always a valid load)
             const int k0 = gi * 8;
             double dx[8], dy[8], dz[8], rs[8];
             int jj[8], tt[8];
@@ -121,12 +121,11 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const bool valid = k0 + u < cnt;
-                const int t = valid ? e.t[u] : 0;
+                const int t = valid ? nb_entry(e, u) : 0;
                 double xj, yj, zj;
                 if (staged) {
-                    const double4 r = sp[t];
-                    xj = r.x; yj = r.y; zj = r.z;
-                    jj[u] = (int)__double_as_longlong(r.w);
+                    xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
+                    jj[u] = sp.j[t];
                 } else {
                     const int kk = win_index(t, W);
                     const double4 r = spos[kk];
@@ -176,7 +175,7 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS
     if (staged) {
         __syncthreads();
         for (int t = threadIdx.x; t < S; t += blockDim.x) {
-            const int j = (int)__double_as_longlong(sp[t].w);
+            const int j = sp.j[t];
             if (j < a.n) {
                 const unsigned long long rx = racc[3 * t + 0], ry = racc[3 * t + 1], rz = racc[3 * t + 2];
                 if ((rx | ry | rz) != 0ull) {
@@ -260,9 +259,9 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARA
         const int w = w0 + at;
         lj_energy_atom<SHIFT>(pos[work[w]], w, nbc[w], nbl, a, half, a.n, staged, sx, sy, sz, sj, spos, parts, W, pe, vir, np);
     }
-    const double bpe = block_sum_double<kTileThreads>(pe);
-    const double bvir = block_sum_double<kTileThreads>(vir);
-    const double bnp = block_sum_double<kTileThreads>(np);
+    const double bpe = block_sum_double_any(pe);
+    const double bvir = block_sum_double_any(vir);
+    const double bnp = block_sum_double_any(np);
     if (threadIdx.x == 0) {
         partials[3 * blockIdx.x + 0] = bpe;
         partials[3 * blockIdx.x + 1] = bvir;
@@ -433,9 +432,9 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
 #define TMD_LJ_FULL(P, SH, U)                                                                                          \
     do {                                                                                                               \
         if (kick_dt > 0.0)                                                                                             \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
         else                                                                                                           \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
     } while (0)
         if (s.precision == PREC_FP64) {
             if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }
@@ -455,11 +454,11 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         unsigned long long* A = s.acc.as<unsigned long long>();
         // window records (32 B) + three 64-bit reaction accumulators per atom
         const int per = kStageBytes + 3 * (int)sizeof(unsigned long long);
-        const int wh = std::min(s.max_window, std::min(s.lds_limit, kLdsHalfBudget) / per);
+        const int wh = std::min(s.max_window, std::min(s.lds_limit, kLdsHalfBudget) / per) & ~1;
         const size_t lds = (size_t)wh * per;
         double4* F = s.force.as<double4>();
 #define TMD_LJ_HALF(P, SH) \
-    hipLaunchKernelGGL((k_lj_half<P, SH>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, a, wh, F, A, flag)
+    hipLaunchKernelGGL((k_lj_half<P, SH>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, wh, F, A, flag)
         if (s.precision == PREC_FP64) {
             if (shift) TMD_LJ_HALF(PREC_FP64, 1); else TMD_LJ_HALF(PREC_FP64, 0);
         } else {
@@ -499,11 +498,11 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
     if (s.precision == PREC_FP64) {
-        if (half) hipLaunchKernelGGL((k_dem<PREC_FP64, 1>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
-        else hipLaunchKernelGGL((k_dem<PREC_FP64, 0>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        if (half) hipLaunchKernelGGL((k_dem<PREC_FP64, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        else hipLaunchKernelGGL((k_dem<PREC_FP64, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
     } else {
-        if (half) hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 1>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
-        else hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 0>), dim3(nb), dim3(kTileThreads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        if (half) hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        else hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
     }
     if (half) hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F);
     s.timer.end(K_FORCE, s.stream, ev);
@@ -516,9 +515,9 @@ void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part)
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     if (!s.mirror_fresh) refresh_mirror(s);
     if (s.lj_shift)
-        hipLaunchKernelGGL(k_lj_energy<1>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
+        hipLaunchKernelGGL(k_lj_energy<1>, dim3(nb), dim3(s.tile_threads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
     else
-        hipLaunchKernelGGL(k_lj_energy<0>, dim3(nb), dim3(kTileThreads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
+        hipLaunchKernelGGL(k_lj_energy<0>, dim3(nb), dim3(s.tile_threads), lds_bytes(s, true), s.stream, TMD_LIST_ARGS, a, s.nb_half, part);
     TMD_HIP(hipGetLastError());
 }
 

@@ -35,8 +35,8 @@ __global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restric
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
-    double4* sp = reinterpret_cast<double4*>(dyn);
-    const bool staged = stage_tile_xyzj(W, a.wcap, spos, parts, sp);
+    const WinStage sp = win_stage(dyn, a.wcap);
+    const bool staged = stage_window(W, a.wcap, spos, parts, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
@@ -118,7 +118,7 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
         const size_t lds = (size_t)wcap * kStageBytes;
         if (s.n > 0)
-            hipLaunchKernelGGL(k_nlist, dim3(ntiles), dim3(kTileThreads), lds, s.stream, (const double4*)s.spos.as<double4>(),
+            hipLaunchKernelGGL(k_nlist, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(),
                                (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(),
                                (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(),
                                (const int*)s.work.as<int>(), (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(),

@@ -611,6 +611,15 @@ void tmd_set_sync(int on) { sim().sync_calls = on ? 1 : 0; }
 void tmd_set_dem_radius(double r) { sim().dem_radius = r; }
 void tmd_set_lj_shift(int on) { sim().lj_shift = on ? 1 : 0; }
 
+void tmd_set_tile_threads(int threads) {
+    Sim& s = sim();
+    if (threads < 64 || threads > kTileThreads || threads % 64 != 0) {
+        set_error(__FILE__, __LINE__, "tmd_set_tile_threads: threads must be a multiple of 64 in [64, 512]");
+        return;
+    }
+    s.tile_threads = threads;
+}
+
 void tmd_set_tile(int tx, int ty, int tz) {
     Sim& s = sim();
     s.tile_req[0] = tx;

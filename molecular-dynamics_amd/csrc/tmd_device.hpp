@@ -66,6 +66,25 @@ __device__ __forceinline__ double block_sum_double(double v) {
     return r;
 }
 
+// Deterministic block sum of doubles for any blockDim.x multiple of 64
+// (fixed shuffle tree, fixed wave order).  Valid in thread 0.
+__device__ __forceinline__ double block_sum_double_any(double v) {
+    __shared__ double wsum[16];  // up to 1024 threads
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_down(v, d, kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    if (lane == 0) wsum[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (int)blockDim.x / kWave;
+        for (int w = 0; w < nw; ++w) r += wsum[w];
+    }
+    __syncthreads();
+    return r;
+}
+
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
 // with a private L2.  Remap so that XCD x processes one contiguous range of
 // logical blocks: atoms that are neighbours in cell order (and so share

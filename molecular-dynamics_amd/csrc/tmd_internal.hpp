@@ -95,6 +95,7 @@ struct Sim {
     int max_window = 0; // largest tile window (atoms)
     TileGeom tiles{};   // tile decomposition of the cell grid
     int tile_req[3] = {0, 0, 0};  // requested cells per tile (0: automatic)
+    int tile_threads = kTileThreads;  // threads per tile block (multiple of 64, <= kTileThreads)
     int windows_valid = 0;
     int inv_valid = 0;      // cell_inv matches the current atom set (binning after the last ghost build)
     int mirror_fresh = 0;   // spos[k] == pos[cell_parts[k]] for every binned atom

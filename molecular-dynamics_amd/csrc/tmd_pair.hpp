@@ -26,17 +26,10 @@ __device__ __forceinline__ float lj_f32(const LJArgs& a, float rsq) {
     return f;
 }
 
-struct Nb8 {
-    int t[8];
-};
-
-__device__ __forceinline__ Nb8 unpack8(const int4 q) {
-    Nb8 r;
-    r.t[0] = q.x & 0xffff; r.t[1] = (unsigned)q.x >> 16;
-    r.t[2] = q.y & 0xffff; r.t[3] = (unsigned)q.y >> 16;
-    r.t[4] = q.z & 0xffff; r.t[5] = (unsigned)q.z >> 16;
-    r.t[6] = q.w & 0xffff; r.t[7] = (unsigned)q.w >> 16;
-    return r;
+// entry u (0..7) of a packed group of 8 u16 list entries
+__device__ __forceinline__ int nb_entry(const int4& q, int u) {
+    const int w = (u >> 1) == 0 ? q.x : ((u >> 1) == 1 ? q.y : ((u >> 1) == 2 ? q.z : q.w));
+    return (u & 1) ? (int)((unsigned)w >> 16) : (w & 0xffff);
 }
 
 
@@ -72,50 +65,62 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 // Neighbour position of window offset t: LDS (x, y, z interleaved) when the
 // window is staged, else the cell-ordered mirror.
 template <bool STAGED>
-__device__ __forceinline__ void win_pos(int t, const double4* __restrict__ sp, const double4* __restrict__ spos, const TileWin& W,
+__device__ __forceinline__ void win_pos(int t, const WinStage& st, const double4* __restrict__ spos, const TileWin& W,
                                         double& x, double& y, double& z) {
-    const double4 p = STAGED ? sp[t] : spos[win_index(t, W)];
-    x = p.x;
-    y = p.y;
-    z = p.z;
+    if (STAGED) {
+        x = st.xyz[3 * t + 0];
+        y = st.xyz[3 * t + 1];
+        z = st.xyz[3 * t + 2];
+    } else {
+        const double4 p = spos[win_index(t, W)];
+        x = p.x;
+        y = p.y;
+        z = p.z;
+    }
 }
 
-// Full-list force on one atom (lane).  Entries are read 8 at a time; the 8
-// pair terms are evaluated without branches (masked entries use a safe
-// rsq and a zero scalar) and accumulated in list order.  Adding a masked
-// term adds dx * 0.0 = +-0.0, which leaves the running sum unchanged (the
-// sum starts at +0.0 and never becomes -0.0), so the FP64 result equals the
+// Full-list force on one atom (lane).  Entries are read 8 at a time (one
+// 16-byte load, prefetched a group ahead) and evaluated 4 at a time; the
+// pair terms are evaluated without branches (a masked entry's scalar is
+// replaced by zero) and accumulated in list order.  Adding a masked term
+// adds dx * 0.0 = +-0.0, which leaves the running sum unchanged (the sum
+// starts at +0.0 and never becomes -0.0), so the FP64 result equals the
 // sequential reference sum bit for bit.
 template <int PREC, int SHIFT, int UNIT, bool STAGED>
-__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const double4* __restrict__ sp,
+__device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const WinStage& sp,
                                                 const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
     const int ng = (cnt + 7) >> 3;
     // the next group of 8 entries is loaded while the current one is used
-    int4 q = ng > 0 ? lp[0] : make_int4(0, 0, 0, 0);
+    int4 q = lp[0];  // group 0 of the column always exists (capacity >= 8)
     for (int gi = 0; gi < ng; ++gi) {
-        const int4 qn = (gi + 1 < ng) ? lp[(size_t)(gi + 1) * 64] : q;
-        const Nb8 e = unpack8(q);
-        q = qn;
-        const int k0 = gi * 8;
-        double dx[8], dy[8], dz[8], rs[8];
-        bool ok[8];
+        const int4 e = q;
+        q = lp[(size_t)min(gi + 1, ng - 1) * 64];  // next group (clamped: always a valid load)
+        // two passes of 4 entries (fewer live registers: 5 waves per SIMD)
+#pragma unroll 1
+        for (int hb = 0; hb < 8; hb += 4) {
+        const int k0 = gi * 8 + hb;
+        double dx[4], dy[4], dz[4], rs[4];
+        bool ok[4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 4; ++u) {
             const bool valid = k0 + u < cnt;
             double xj, yj, zj;
-            win_pos<STAGED>(valid ? e.t[u] : 0, sp, spos, W, xj, yj, zj);
+            win_pos<STAGED>(valid ? nb_entry(e, hb + u) : 0, sp, spos, W, xj, yj, zj);
             dx[u] = pi.x - xj;
             dy[u] = pi.y - yj;
             dz[u] = pi.z - zj;
-            const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-            ok[u] = valid && rsq < a.rc2;
-            rs[u] = ok[u] ? rsq : 1.0;
+            rs[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            ok[u] = valid && rs[u] < a.rc2;
         }
+        // The pair scalar is evaluated for every entry and masked afterwards:
+        // a masked entry may have rsq = 0 (its own position) or lie beyond
+        // the cutoff; its (possibly infinite) scalar is replaced by 0 by the
+        // select, so the masked term adds dx * 0 = +-0.
         if (PREC == PREC_FP64) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 const double f = lj_f64u<SHIFT, UNIT>(a, rs[u]);
                 const double fm = ok[u] ? f : 0.0;
                 fx = fx + dx[u] * fm;
@@ -123,26 +128,28 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
                 fz = fz + dz[u] * fm;
             }
         } else if (PREC == PREC_FP32_MIXED) {
-            // FP32 pair scalar, FP64 accumulation of dx * f (each FP32 term is
-            // exact in FP64; the 29 extra mantissa bits absorb the sum's rounding)
+            // FP32 pair scalar, FP64 fused multiply-add accumulation of dx * f
+            // (one rounding per term; the FP64 mantissa absorbs the sum's
+            // rounding)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
                 const double fm = (double)(ok[u] ? f : 0.0f);
-                fx = fx + dx[u] * fm;
-                fy = fy + dy[u] * fm;
-                fz = fz + dz[u] * fm;
+                fx = fma(dx[u], fm, fx);
+                fy = fma(dy[u], fm, fy);
+                fz = fma(dz[u], fm, fz);
             }
         } else {
             // FP32 pair terms with Kahan-compensated FP32 accumulation
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
                 const float fm = ok[u] ? f : 0.0f;
                 kahan_add(sxa, cxa, (float)dx[u] * fm);
                 kahan_add(sya, cya, (float)dy[u] * fm);
                 kahan_add(sza, cza, (float)dz[u] * fm);
             }
+        }
         }
     }
     if (PREC != PREC_FP32_KAHAN) return make_double4(fx, fy, fz, 0.0);
@@ -200,14 +207,14 @@ struct NbArgs {
 };
 
 // Verlet list of local atom i (work slot w, cell c of the tile's grid) from
-// the staged window (x, y, z, j records): 9 stencil rows of 3 cells in the
+// the staged window: 9 stencil rows of 3 cells in the
 // reference's order, `rsq <= cut2` in FP64 without contraction.  Accepted
 // window offsets are packed 8 to a 16-byte group in registers and each full
 // group is written with one store (one list column per lane, so a wave's
 // stores fall in one 1 KiB stripe).  Returns the count; groups beyond the
 // capacity are counted, not stored.
 __device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi, const int* __restrict__ start,
-                                          const NbArgs& a, const TileWin& W, bool staged, const double4* sp,
+                                          const NbArgs& a, const TileWin& W, bool staged, const WinStage& sp,
                                           const double4* __restrict__ spos, const int* __restrict__ parts,
                                           nb_t* __restrict__ nb_list) {
     const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
@@ -228,9 +235,8 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi,
                 int j;
                 double xj, yj, zj;
                 if (staged) {
-                    const double4 q = sp[t];
-                    xj = q.x; yj = q.y; zj = q.z;
-                    j = (int)__double_as_longlong(q.w);
+                    xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
+                    j = sp.j[t];
                 } else {
                     const int kk = win_index(t, W);
                     j = parts[kk];

@@ -27,7 +27,7 @@ constexpr int kMaxTile = 8;        // cells per tile dimension, at most
 constexpr int kMaxSeg = (kMaxTile + 2) * (kMaxTile + 2);
 constexpr int kWinMax = 65535;     // window offsets are 16-bit
 constexpr int kLdsBudget = 80 * 1024;  // LDS per block: two blocks per 160 KiB CU
-constexpr int kStageBytes = 32;       // LDS bytes per staged window atom (a double4 record)
+constexpr int kStageBytes = 28;       // LDS bytes per staged window atom: x, y, z 24, index 4
 constexpr int kLdsHalfBudget = 144 * 1024;  // half-list LJ: window records + reaction accumulators (one block per CU)
 
 typedef unsigned short nb_t;
@@ -131,32 +131,37 @@ __device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const dou
     return staged;
 }
 
-// Stage the window as one 32-byte record per atom, (x, y, z, atom index as
-// the bit pattern of the fourth double), read back with two 16-byte LDS
-// loads.  Whole block.  Returns whether it was staged (S <= wcap).
-__device__ __forceinline__ bool stage_tile_xyzj(const TileWin& W, int wcap, const double4* __restrict__ spos,
-                                                const int* __restrict__ parts, double4* sp) {
+// LDS image of a tile's window: x, y, z interleaved (three 8-byte loads per
+// neighbour; measured faster than 16-byte loads for these scattered
+// accesses) and optionally the atom index; kStageBytes per atom.
+struct WinStage {
+    double* xyz;
+    int* j;
+};
+
+__device__ __forceinline__ WinStage win_stage(double* dyn, int wcap) {
+    WinStage st;
+    st.xyz = dyn;
+    st.j = reinterpret_cast<int*>(dyn + 3 * wcap);
+    return st;
+}
+
+// Stage the window (positions, and atom indices when parts != nullptr) into
+// LDS if it fits wcap atoms.  Whole block.  Returns whether it was staged;
+// otherwise consumers read the mirror.
+__device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const double4* __restrict__ spos,
+                                             const int* __restrict__ parts, const WinStage& st) {
     const int S = win_size(W);
     const bool staged = S <= wcap;
     if (staged) {
         for (int t = threadIdx.x; t < S; t += blockDim.x) {
             const int k = win_index(t, W);
             const double4 p = spos[k];
-            sp[t] = make_double4(p.x, p.y, p.z, __longlong_as_double((long long)parts[k]));
+            st.xyz[3 * t + 0] = p.x;
+            st.xyz[3 * t + 1] = p.y;
+            st.xyz[3 * t + 2] = p.z;
+            if (parts) st.j[t] = parts[k];
         }
-    }
-    __syncthreads();
-    return staged;
-}
-
-// Stage the window's FP64 positions as double4 records into LDS (two
-// 16-byte loads per neighbour) if it fits wcap atoms.  Whole block.
-// Returns whether it was staged.
-__device__ __forceinline__ bool stage_tile_aos(const TileWin& W, int wcap, const double4* __restrict__ spos, double4* sp) {
-    const int S = win_size(W);
-    const bool staged = S <= wcap;
-    if (staged) {
-        for (int t = threadIdx.x; t < S; t += blockDim.x) sp[t] = spos[win_index(t, W)];
     }
     __syncthreads();
     return staged;

@@ -79,6 +79,7 @@ SIGNATURES = {
     "tmd_set_dem_radius": (None, [_d]),
     "tmd_set_lj_shift": (None, [_i]),
     "tmd_set_tile": (None, [_i, _i, _i]),
+    "tmd_set_tile_threads": (None, [_i]),
     "tmd_thermo": (None, [_d, _d, _d, _dp]),
     "tmd_get_grid_info": (None, [_ip]),
     "tmd_get_ghost_table": (_i, [_ip, _ip, _ip, _ip]),

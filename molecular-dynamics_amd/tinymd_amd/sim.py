@@ -93,6 +93,16 @@ class Simulation:
     def set_precision(self, precision):
         self.L.tmd_set_precision(PRECISION[precision])
 
+    def set_tile(self, tx=0, ty=None, tz=None, threads=None):
+        """Cells per tile (0: automatic) and threads per tile block; applies
+        from the next binning."""
+        ty = tx if ty is None else ty
+        tz = tx if tz is None else tz
+        self.L.tmd_set_tile(tx, ty, tz)
+        if threads is not None:
+            self.L.tmd_set_tile_threads(threads)
+        check()
+
     @classmethod
     def fcc(cls, nx, ny=None, nz=None, potmin=1.6796, temp=1.44, half_benchmark=False, cutoff=2.5,
             skin=0.3, **kw):
