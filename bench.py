@@ -34,14 +34,16 @@ RC, SKIN, DT, REN = 2.5, 0.3, 0.005, 20
 
 
 def force_kernel_bytes(sim, precision):
-    """Algorithmic HBM bytes of one pair-force launch (full list): neighbour
-    counts and work indices (4 B each per atom), 4 B per list entry, each
-    position read once (24 B, x/y/z FP64), each force written once (24 B)."""
+    """Algorithmic HBM bytes of one step-loop pair-force launch (full list,
+    final kick fused): neighbour counts and work indices (4 B each per
+    atom), 4 B per list entry (the reference's i32 list; this build stores
+    2 B), each position read once (24 B, x/y/z FP64), each force written
+    once (24 B), each velocity read and written once by the kick (48 B)."""
     counts, _ = sim.neighbor_lists(with_lists=False)
     n = sim.nlocal
     nall = n + sim.nghost
     entries = int(counts.astype(np.int64).sum())
-    return 4 * n + 4 * n + 4 * entries + 24 * nall + 24 * n, entries
+    return 4 * n + 4 * n + 4 * entries + 24 * nall + 24 * n + 48 * n, entries
 
 
 def run_variant(tm, precision, half, steps, warmup, profile=False):
@@ -226,10 +228,14 @@ def copy_calibration(nbytes=1 << 30, reps=5):
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "force_lj_fp32mixed_80cube_traffic.json")
+    """HBM bytes per launch of the force kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    import re
+    p = os.path.join(ROOT, "profiles", "force_lj_fp32mixed_80cube_traffic.md")
     if os.path.exists(p):
-        with open(p) as fh:
-            return json.load(fh).get("traffic_bytes_per_launch")
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(p).read(), re.M)
+        if m:
+            return int(m.group(1))
     return None
 
 
@@ -316,7 +322,7 @@ def main():
                    "atoms": n, "list_entries": entries},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_lj_full_f32", "bytes_per_launch": nbytes,
+                     "kernel": "k_lj_full<FP32-mixed, kick fused>", "bytes_per_launch": nbytes,
                      "avg_launch_ms": r32["force_ms_avg"]},
         "fp32_vs_fp64_force_err": {"rms_rel": err_rms, "max_abs_over_rms": err_max},
         "pair_interactions_per_s": value * pairs_per_atom,
