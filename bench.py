@@ -47,15 +47,15 @@ def force_kernel_bytes(sim, precision):
 
 
 def run_variant(tm, precision, half, steps, warmup, profile=False):
+    """Timed run of `steps` steps (no per-kernel timers), then, with
+    profile, a second run of the same length with HIP-event timers on the
+    library stream for per-kernel averages."""
     sim = tm.Simulation.fcc(NC, precision=precision)
     sim.setup(half, RC, SKIN)
     sim.lennard_jones(half, RC)
     if warmup:
         sim.run_lj(warmup, REN, DT, half, RC, SKIN)
     sim.barrier()
-    if profile:
-        sim.profile(True)
-        sim.profile_reset()
     t0 = time.perf_counter()
     done = sim.run_lj(steps, REN, DT, half, RC, SKIN)
     sim.barrier()
@@ -63,6 +63,10 @@ def run_variant(tm, precision, half, steps, warmup, profile=False):
     assert done == steps
     out = {"seconds": t1 - t0, "atom_steps_per_s": sim.nlocal * steps / (t1 - t0)}
     if profile:
+        sim.profile(True)
+        sim.profile_reset()
+        sim.run_lj(steps, REN, DT, half, RC, SKIN)
+        sim.barrier()
         ms, cnt = sim.profile_get("force")
         out["force_ms_avg"] = ms / max(cnt, 1)
         out["force_launches"] = cnt

@@ -25,6 +25,7 @@
 // tile size and block size.  The cell spacing changes list order, so it
 // keeps results identical only to single-box runs with the same spacing.
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <map>
 #include <memory>
@@ -44,7 +45,6 @@ struct RepDesc {
     int sb0, nsb;                // scan blocks (kScanTile slots) over [0, cap)
     int eb0, neb;                // 256-blocks over the locals
     int cb0, ncb;                // 256-blocks over [0, cap)
-    int gb0, ngb;                // 256-blocks over the ghost room
     TileGeom g;
     double aabb[6], len[3], spacing;
     double lo_pbc[3], hi_pbc[3];     // world bounds (periodic wrap)
@@ -251,24 +251,29 @@ __global__ void __launch_bounds__(kScanThreads) k_b_border_scatter(const RepDesc
     }
 }
 
-// per-step ghost refresh: ghost = owner + image * L
-__global__ void k_b_refresh(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep, const int* __restrict__ dyn,
-                            double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
-                            const int* __restrict__ inv, double4* __restrict__ spos) {
-    const int r = gb_rep[blockIdx.x];
+// after a force pass that advanced the locals: fresh ghosts and the mirror
+// of every binned local and ghost
+__global__ void k_b_step_refresh(const RepDesc* __restrict__ R, const int* __restrict__ cb_rep, const int* __restrict__ dyn,
+                                 double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
+                                 const int* __restrict__ inv, double4* __restrict__ spos) {
+    const int r = cb_rep[blockIdx.x];
     const RepDesc& D = R[r];
-    const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
-    if (g >= dyn[r * DY_STRIDE + DY_NG]) return;
-    const int s = D.a0 + D.n + g;
-    int ix, iy, iz;
-    b_img_decode(g_code[s], ix, iy, iz);
-    const double4 p = pos[D.a0 + g_owner[s]];
+    const int i = (blockIdx.x - D.cb0) * kBlk + threadIdx.x;
+    if (i >= D.n + dyn[r * DY_STRIDE + DY_NG]) return;
+    const int s = D.a0 + i;
     double4 q;
-    q.x = p.x + D.len[0] * (double)ix;
-    q.y = p.y + D.len[1] * (double)iy;
-    q.z = p.z + D.len[2] * (double)iz;
-    q.w = p.w;
-    pos[s] = q;
+    if (i < D.n) {
+        q = pos[s];
+    } else {
+        int ix, iy, iz;
+        b_img_decode(g_code[s], ix, iy, iz);
+        const double4 p = pos[D.a0 + g_owner[s]];
+        q.x = p.x + D.len[0] * (double)ix;
+        q.y = p.y + D.len[1] * (double)iy;
+        q.z = p.z + D.len[2] * (double)iz;
+        q.w = p.w;
+        pos[s] = q;
+    }
     const int k = inv[s];
     if (k >= 0) spos[k] = q;
 }
@@ -406,12 +411,14 @@ __global__ void k_b_mirror(const double4* __restrict__ pos, const int* __restric
 // neighbour lists and forces: one block per tile of any replica
 __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                           const double4* __restrict__ spos, const double4* __restrict__ pos,
-                                                          const int* __restrict__ cell_of, const int* __restrict__ start,
+                                                          const int* __restrict__ cell_of, const int* __restrict__ inv,
+                                                          const int* __restrict__ start,
                                                           const int* __restrict__ parts, const int* __restrict__ work,
                                                           const int* __restrict__ tile_start, NbArgs a,
                                                           int* __restrict__ nb_count, nb_t* __restrict__ nb_list,
                                                           int* __restrict__ flags) {
     extern __shared__ double dyn[];
+    int* cst = reinterpret_cast<int*>(reinterpret_cast<char*>(dyn) + (size_t)kStageBytes * a.wcap);
     const int tile = blockIdx.x;
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
@@ -422,24 +429,27 @@ __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restr
     const int* st = start + D.c0;
     __shared__ TileWin W;
     tile_window(tile - D.t0, st, D.g, W);
+    stage_row_starts(W, st, D.g, cst);
     const WinStage sp = win_stage(dyn, a.wcap);
-    const bool staged = stage_window(W, a.wcap, spos, parts, sp);
+    const bool staged = stage_window(W, a.wcap, spos, nullptr, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], st, aa, W, staged, sp, spos, parts, nb_list);
+        const int cnt = nlist_atom<false>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, staged, sp, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(&flags[FL_LIST], cnt);
     }
 }
 
-template <int PREC, int SHIFT, int UNIT>
+// ADV 1: also the next step's first kick and drift (as k_lj_full<..., 2>)
+template <int PREC, int SHIFT, int UNIT, int ADV>
 __global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                             const double4* __restrict__ pos, const double4* __restrict__ spos,
                                                             const int* __restrict__ start, const int* __restrict__ work,
                                                             const int* __restrict__ tile_start, const int* __restrict__ nbc,
                                                             const nb_t* __restrict__ nbl, LJArgs a, double4* __restrict__ force,
-                                                            double4* __restrict__ vel, double hdt) {
+                                                            double4* __restrict__ vel, double hdt, double dt,
+                                                            double4* __restrict__ pos_out) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -463,6 +473,17 @@ __global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __res
         v.x = v.x + f.x * hdt;
         v.y = v.y + f.y * hdt;
         v.z = v.z + f.z * hdt;
+        if (ADV) {
+            // next step's k_b_int1
+            v.x = v.x + f.x * hdt;
+            v.y = v.y + f.y * hdt;
+            v.z = v.z + f.z * hdt;
+            double4 x = pi;
+            x.x = x.x + v.x * dt;
+            x.y = x.y + v.y * dt;
+            x.z = x.z + v.z * dt;
+            pos_out[i] = x;
+        }
         vel[i] = v;
     }
 }
@@ -559,21 +580,26 @@ struct Batch {
     double ghost_scale = 1.0;        // grows after a ghost overflow
 
     std::vector<RepDesc> R;
-    int n_sb = 0, n_eb = 0, n_cb = 0, n_gb = 0, n_tiles = 0, n_cells = 0, n_slots = 0, n_work = 0;
+    int n_sb = 0, n_eb = 0, n_cb = 0, n_tiles = 0, n_cells = 0, n_slots = 0, n_work = 0;
     int list_cap = 8, wcap = 0;
-    DevBuf dR, sb_rep, eb_rep, cb_rep, gb_rep, tile_rep, cell_rep;
+    DevBuf dR, sb_rep, eb_rep, cb_rep, tile_rep, cell_rep;
     DevBuf pos, vel, force, g_owner, g_code, cell_of, cell_slot, cell_inv, parts, spos;
     DevBuf cell_count, cell_start, tile_count, tile_start, work;
     DevBuf sb_cnt0, sb_cnt1, scan_tmp, nb_count, nb_list;
     DevBuf dyn, flags, epart, kpart, tout;
     DevBuf snap_pos, snap_vel, snap_force;
     hipStream_t stream = nullptr;
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    struct CapturedRun {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        std::vector<unsigned long long> sig;  // pointers and scalars baked into the graph
+    };
+    std::map<int, CapturedRun> graphs;
     bool ready = false;
 
     ~Batch() {
         drop_graphs();
-        DevBuf* bufs[] = {&dR, &sb_rep, &eb_rep, &cb_rep, &gb_rep, &tile_rep, &cell_rep, &pos, &vel, &force, &g_owner,
+        DevBuf* bufs[] = {&dR, &sb_rep, &eb_rep, &cb_rep, &tile_rep, &cell_rep, &pos, &vel, &force, &g_owner,
                           &g_code, &cell_of, &cell_slot, &cell_inv, &parts, &spos, &cell_count, &cell_start, &tile_count,
                           &tile_start, &work, &sb_cnt0, &sb_cnt1, &scan_tmp, &nb_count, &nb_list, &dyn, &flags, &epart,
                           &kpart, &tout, &snap_pos, &snap_vel, &snap_force};
@@ -583,8 +609,8 @@ struct Batch {
     }
     void drop_graphs() {
         for (auto& kv : graphs) {
-            (void)hipGraphExecDestroy(kv.second.second);
-            (void)hipGraphDestroy(kv.second.first);
+            (void)hipGraphExecDestroy(kv.second.exec);
+            (void)hipGraphDestroy(kv.second.graph);
         }
         graphs.clear();
     }
@@ -606,11 +632,10 @@ static int scan_scratch(int n) { return div_up(std::max(n, 1), kScanTile) + 1; }
 
 // Descriptors, tables and buffers for the current configuration.
 static void layout(Batch& B) {
-    B.drop_graphs();
     B.R.assign(B.nrep, RepDesc{});
     const double cut = B.rc + B.skin;
     const double spacing = cut * B.cfg.spacing;
-    std::vector<int> sb, eb, cb, gb, tr, cr;
+    std::vector<int> sb, eb, cb, tr, cr;
     int a0 = 0, w0 = 0, c0 = 0, t0 = 0;
     for (int r = 0; r < B.nrep; ++r) {
         RepDesc& D = B.R[r];
@@ -637,11 +662,9 @@ static void layout(Batch& B) {
         D.sb0 = (int)sb.size(); D.nsb = div_up(D.cap, kScanTile);
         D.eb0 = (int)eb.size(); D.neb = std::max(1, div_up(D.n, kBlk));
         D.cb0 = (int)cb.size(); D.ncb = div_up(D.cap, kBlk);
-        D.gb0 = (int)gb.size(); D.ngb = std::max(1, div_up(gcap, kBlk));
         for (int k = 0; k < D.nsb; ++k) sb.push_back(r);
         for (int k = 0; k < D.neb; ++k) eb.push_back(r);
         for (int k = 0; k < D.ncb; ++k) cb.push_back(r);
-        for (int k = 0; k < D.ngb; ++k) gb.push_back(r);
         for (int k = 0; k < D.ntiles; ++k) tr.push_back(r);
         for (int k = 0; k < D.ncells; ++k) cr.push_back(r);
         for (int d = 0; d < 6; ++d) D.aabb[d] = G.aabb[d];
@@ -668,7 +691,6 @@ static void layout(Batch& B) {
     B.n_sb = (int)sb.size();
     B.n_eb = (int)eb.size();
     B.n_cb = (int)cb.size();
-    B.n_gb = (int)gb.size();
     hipStream_t st = B.stream;
     auto upload_ints = [&](DevBuf& b, const std::vector<int>& v) {
         b.reserve(sizeof(int) * std::max<size_t>(v.size(), 1), false, st);
@@ -679,7 +701,6 @@ static void layout(Batch& B) {
     upload_ints(B.sb_rep, sb);
     upload_ints(B.eb_rep, eb);
     upload_ints(B.cb_rep, cb);
-    upload_ints(B.gb_rep, gb);
     upload_ints(B.tile_rep, tr);
     upload_ints(B.cell_rep, cr);
     const size_t S = (size_t)B.n_slots;
@@ -796,38 +817,52 @@ static void enq_nlist(Batch& B) {
     a.cap = B.list_cap;
     a.half = 0;
     a.half_mode = HALF_GHOST_FULL;
-    a.wcap = B.wcap;
+    // row starts of the largest tile shape among the replicas
+    int rsb = 0;
+    for (const RepDesc& D : B.R) rsb = std::max(rsb, row_starts_bytes(D.g));
+    a.wcap = std::min(B.wcap, (kLdsBudget - rsb) / kStageBytes);
     const double cut = B.rc + B.skin;
     a.cut2 = cut * cut;
-    const size_t lds = (size_t)a.wcap * kStageBytes;
+    const size_t lds = (size_t)a.wcap * kStageBytes + rsb;
     hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.tile_rep.as<int>(), (const double4*)B.spos.as<double4>(), (const double4*)B.pos.as<double4>(),
-                       (const int*)B.cell_of.as<int>(), (const int*)B.cell_start.as<int>(), (const int*)B.parts.as<int>(),
+                       (const int*)B.cell_of.as<int>(), (const int*)B.cell_inv.as<int>(), (const int*)B.cell_start.as<int>(),
+                       (const int*)B.parts.as<int>(),
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), a, B.nb_count.as<int>(),
                        B.nb_list.as<nb_t>(), B.flags.as<int>());
     TMD_HIP(hipGetLastError());
 }
 
-static void enq_refresh(Batch& B) {
-    hipLaunchKernelGGL(k_b_refresh, dim3(B.n_gb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.gb_rep.as<int>(), (const int*)B.dyn.as<int>(), B.pos.as<double4>(),
+static void enq_step_refresh(Batch& B) {
+    hipLaunchKernelGGL(k_b_step_refresh, dim3(B.n_cb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
+                       (const int*)B.cb_rep.as<int>(), (const int*)B.dyn.as<int>(), B.pos.as<double4>(),
                        (const int*)B.g_owner.as<int>(), (const int*)B.g_code.as<int>(), (const int*)B.cell_inv.as<int>(),
                        B.spos.as<double4>());
 }
 
-static void enq_force(Batch& B, bool kick) {
+// kick: final kick v += F dt/2 (hdt = 0 leaves v unchanged: v + f * 0 == v);
+// advance: also the next step's kick and drift
+static void enq_force(Batch& B, bool kick, bool advance) {
     const LJArgs a = batch_lj_args(B);
     const size_t lds = (size_t)B.wcap * kStageBytes;
     const bool unit = B.eps == 1.0 && B.sigma == 1.0;
-    const double hdt = kick ? 0.5 * B.dt : 0.0;  // hdt = 0 leaves v unchanged (v + f * 0 == v)
+    const double hdt = kick ? 0.5 * B.dt : 0.0;
 #define TMD_B_LJ(P, SH, U)                                                                                          \
-    hipLaunchKernelGGL((k_b_lj_full<P, SH, U>), dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,                 \
+    hipLaunchKernelGGL((k_b_lj_full<P, SH, U, 1>), dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,              \
                        (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
                        (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), \
                        (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), (const nb_t*)B.nb_list.as<nb_t>(), \
-                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt)
+                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, B.pos.as<double4>())
+#define TMD_B_LJ0(P, SH, U)                                                                                         \
+    hipLaunchKernelGGL((k_b_lj_full<P, SH, U, 0>), dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,              \
+                       (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
+                       (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), \
+                       (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), (const nb_t*)B.nb_list.as<nb_t>(), \
+                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, B.pos.as<double4>())
+#define TMD_B_LJ_A(P, SH, U) \
+    do { if (advance) TMD_B_LJ(P, SH, U); else TMD_B_LJ0(P, SH, U); } while (0)
 #define TMD_B_LJ_U(P, SH) \
-    do { if (unit) TMD_B_LJ(P, SH, 1); else TMD_B_LJ(P, SH, 0); } while (0)
+    do { if (unit) TMD_B_LJ_A(P, SH, 1); else TMD_B_LJ_A(P, SH, 0); } while (0)
 #define TMD_B_LJ_S(P) \
     do { if (B.shift) TMD_B_LJ_U(P, 1); else TMD_B_LJ_U(P, 0); } while (0)
     if (B.precision == PREC_FP64) TMD_B_LJ_S(PREC_FP64);
@@ -835,6 +870,8 @@ static void enq_force(Batch& B, bool kick) {
     else TMD_B_LJ_S(PREC_FP32_MIXED);
 #undef TMD_B_LJ_S
 #undef TMD_B_LJ_U
+#undef TMD_B_LJ_A
+#undef TMD_B_LJ0
 #undef TMD_B_LJ
     TMD_HIP(hipGetLastError());
 }
@@ -846,18 +883,20 @@ static void enq_int1(Batch& B) {
                        B.spos.as<double4>());
 }
 
-// the main.cpp step loop over `steps` steps (rebuild when (j + 1) % reneigh == 0)
+// the main.cpp step loop over `steps` steps (rebuild when (j + 1) % reneigh
+// == 0); each force pass also performs the final kick of its step and the
+// first kick + drift of the next, so only the first step integrates alone
 static void enq_steps(Batch& B, int steps) {
+    if (steps > 0) enq_int1(B);
     for (int j = 0; j < steps; ++j) {
-        enq_int1(B);
         if ((j + 1) % B.reneigh == 0) {
             enq_rebuild(B);
             enq_mirror(B);
             enq_nlist(B);
         } else {
-            enq_refresh(B);  // integration and refresh keep the mirror current
+            enq_step_refresh(B);
         }
-        enq_force(B, true);
+        enq_force(B, true, j + 1 < steps);
     }
     TMD_HIP(hipGetLastError());
 }
@@ -920,7 +959,8 @@ static bool setup_lists(Batch& B) {
             return false;
         }
         const int maxwin = fl[FL_MAXWIN];
-        B.wcap = std::min(maxwin + maxwin / 4 + 64, kLdsBudget / kStageBytes);
+        // capacities only grow, so a restart keeps a captured run valid
+        B.wcap = std::max(B.wcap, std::min(maxwin + maxwin / 4 + 64, kLdsBudget / kStageBytes));
         enq_mirror(B);
         for (int k = 0; k < 4 && !has_error(); ++k) {
             reserve_lists(B);
@@ -928,11 +968,11 @@ static bool setup_lists(Batch& B) {
             enq_nlist(B);
             read_flags(B, fl);
             if (fl[FL_LIST] == 0) break;
-            B.list_cap = ((int)std::ceil(fl[FL_LIST] * B.cfg.list_headroom) + 7) & ~7;
+            B.list_cap = std::max(B.list_cap, ((int)std::ceil(fl[FL_LIST] * B.cfg.list_headroom) + 7) & ~7);
         }
         if (has_error()) return false;
         if (fl[FL_LIST] > 0) continue;
-        enq_force(B, false);
+        enq_force(B, false, false);
         TMD_HIP(hipStreamSynchronize(B.stream));
         B.ready = !has_error();
         return B.ready;
@@ -941,20 +981,50 @@ static bool setup_lists(Batch& B) {
     return false;
 }
 
-static bool setup_from_initial(Batch& B) {
+// fresh: size the list capacity and the LDS window anew (new configuration);
+// otherwise keep the (only growing) capacities of earlier runs
+static bool setup_from_initial(Batch& B, bool fresh = true) {
     B.ready = false;
     layout(B);
     if (has_error()) return false;
-    // list capacity from the configured headroom over the measured maximum
-    B.list_cap = 8;
+    if (fresh) {
+        B.list_cap = 8;
+        B.wcap = 0;
+    }
     upload_initial(B);
     if (has_error()) return false;
     return setup_lists(B);
 }
 
+// Everything a captured run bakes in: buffer addresses and launch scalars.
+static std::vector<unsigned long long> graph_signature(const Batch& B) {
+    std::vector<unsigned long long> s;
+    const DevBuf* bufs[] = {&B.dR, &B.sb_rep, &B.eb_rep, &B.cb_rep, &B.tile_rep, &B.cell_rep, &B.pos, &B.vel, &B.force,
+                            &B.g_owner, &B.g_code, &B.cell_of, &B.cell_slot, &B.cell_inv, &B.parts, &B.spos,
+                            &B.cell_count, &B.cell_start, &B.tile_count, &B.tile_start, &B.work, &B.sb_cnt0, &B.sb_cnt1,
+                            &B.scan_tmp, &B.nb_count, &B.nb_list, &B.dyn, &B.flags};
+    for (const DevBuf* b : bufs) s.push_back((unsigned long long)(uintptr_t)b->p);
+    const double dv[] = {B.rc, B.skin, B.eps, B.sigma, B.dt};
+    for (double d : dv) {
+        unsigned long long u;
+        std::memcpy(&u, &d, sizeof u);
+        s.push_back(u);
+    }
+    const int iv[] = {B.reneigh, B.precision, B.shift, B.list_cap, B.wcap, B.cfg.threads, B.n_sb, B.n_eb, B.n_cb,
+                      B.n_tiles, B.n_cells, B.n_slots, B.nrep};
+    for (int v : iv) s.push_back((unsigned long long)(unsigned)v);
+    return s;
+}
+
 static hipGraphExec_t get_graph(Batch& B, int steps) {
+    const std::vector<unsigned long long> sig = graph_signature(B);
     auto it = B.graphs.find(steps);
-    if (it != B.graphs.end()) return it->second.second;
+    if (it != B.graphs.end()) {
+        if (it->second.sig == sig) return it->second.exec;
+        (void)hipGraphExecDestroy(it->second.exec);
+        (void)hipGraphDestroy(it->second.graph);
+        B.graphs.erase(it);
+    }
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     TMD_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
@@ -967,7 +1037,7 @@ static hipGraphExec_t get_graph(Batch& B, int steps) {
         (void)hipGraphDestroy(g);
         return nullptr;
     }
-    B.graphs[steps] = {g, ge};
+    B.graphs[steps] = Batch::CapturedRun{g, ge, sig};
     return ge;
 }
 
@@ -1246,8 +1316,7 @@ int tmd_batch_autotune(int h, int trial_steps, double* out) {
 int tmd_batch_reset(int h) {
     Batch* B = find_batch(h);
     if (!B || has_error()) return 0;
-    B->ghost_scale = 1.0;
-    return setup_from_initial(*B) ? 1 : 0;
+    return setup_from_initial(*B, false) ? 1 : 0;
 }
 
 int tmd_batch_get_state(int h, int rep, struct Vector3D* pos, struct Vector3D* vel, struct Vector3D* force) {

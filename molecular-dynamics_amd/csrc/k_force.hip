@@ -39,9 +39,16 @@ namespace tmd {
         const int *__restrict__ parts, const int *__restrict__ work, const int *__restrict__ tile_start,           \
         const int *__restrict__ nbc, const nb_t *__restrict__ nbl
 
+// KICK 0: force only.  KICK 1: plus the final velocity-Verlet kick
+// v += F dt/2.  KICK 2: plus the next step's first kick and drift,
+// v += F dt/2, x += v dt, so the step loop needs no separate integration
+// pass (the lane's own position is read and written only by its lane; the
+// neighbours come from the cell-ordered mirror, which this kernel does not
+// touch).  The arithmetic is that of the separate integration kernels.
 template <int PREC, int SHIFT, int UNIT, int KICK>
 __global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
-                                                         double4* __restrict__ vel, double hdt) {
+                                                         double4* __restrict__ vel, double hdt, double dt,
+                                                         double4* __restrict__ pos_out) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -65,6 +72,17 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS
             v.x = v.x + f.x * hdt;
             v.y = v.y + f.y * hdt;
             v.z = v.z + f.z * hdt;
+            if (KICK == 2) {
+                // next step's k_initial_integration
+                v.x = v.x + f.x * hdt;
+                v.y = v.y + f.y * hdt;
+                v.z = v.z + f.z * hdt;
+                double4 x = pi;
+                x.x = x.x + v.x * dt;
+                x.y = x.y + v.y * dt;
+                x.z = x.z + v.z * dt;
+                pos_out[i] = x;
+            }
             vel[i] = v;
         }
     }
@@ -188,11 +206,11 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS
     }
 }
 
-// force = own sum + accumulated reactions; clears the accumulators; with
-// KICK, the velocity-Verlet final kick v += F dt/2
+// force = own sum + accumulated reactions; clears the accumulators; KICK
+// as in k_lj_full
 template <int KICK>
 __global__ void k_half_finish(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force,
-                              double4* __restrict__ vel, double hdt) {
+                              double4* __restrict__ vel, double hdt, double dt, double4* __restrict__ pos) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long long ax = (long long)acc[3 * (size_t)i + 0];
@@ -211,6 +229,16 @@ __global__ void k_half_finish(unsigned long long* __restrict__ acc, int n, doubl
         v.x = v.x + f.x * hdt;
         v.y = v.y + f.y * hdt;
         v.z = v.z + f.z * hdt;
+        if (KICK == 2) {
+            v.x = v.x + f.x * hdt;
+            v.y = v.y + f.y * hdt;
+            v.z = v.z + f.z * hdt;
+            double4 x = pos[i];
+            x.x = x.x + v.x * dt;
+            x.y = x.y + v.y * dt;
+            x.z = x.z + v.z * dt;
+            pos[i] = x;
+        }
         vel[i] = v;
     }
 }
@@ -410,7 +438,7 @@ static void check_fix_flag(Sim& s) {
 
 static size_t lds_bytes(const Sim& s, bool) { return (size_t)window_cap(s) * kStageBytes; }
 
-void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt) {
+void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt, bool advance) {
     if (has_error()) return;
     TMD_REQUIRE(s.nb_valid, "compute forces: no neighbour list (call md_assemble_neighborlists first)");
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
@@ -429,12 +457,18 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
         const double hdt = 0.5 * kick_dt;
         double4* V = s.vel.as<double4>();
+        double4* X = s.pos.as<double4>();
 #define TMD_LJ_FULL(P, SH, U)                                                                                          \
     do {                                                                                                               \
-        if (kick_dt > 0.0)                                                                                             \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
+        if (kick_dt > 0.0 && advance)                                                                                  \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 2>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
+                               kick_dt, X);                                                                            \
+        else if (kick_dt > 0.0)                                                                                        \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
+                               kick_dt, X);                                                                            \
         else                                                                                                           \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt); \
+            hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
+                               kick_dt, X);                                                                            \
     } while (0)
         if (s.precision == PREC_FP64) {
             if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }
@@ -465,15 +499,19 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
             if (shift) TMD_LJ_HALF(PREC_FP32_MIXED, 1); else TMD_LJ_HALF(PREC_FP32_MIXED, 0);
         }
 #undef TMD_LJ_HALF
-        if (kick_dt > 0.0)
-            hipLaunchKernelGGL(k_half_finish<1>, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F, s.vel.as<double4>(),
-                               0.5 * kick_dt);
+        double4* V = s.vel.as<double4>();
+        double4* X = s.pos.as<double4>();
+        const int nb256 = div_up(s.n, 256);
+        if (kick_dt > 0.0 && advance)
+            hipLaunchKernelGGL(k_half_finish<2>, dim3(nb256), dim3(256), 0, s.stream, A, s.n, F, V, 0.5 * kick_dt, kick_dt, X);
+        else if (kick_dt > 0.0)
+            hipLaunchKernelGGL(k_half_finish<1>, dim3(nb256), dim3(256), 0, s.stream, A, s.n, F, V, 0.5 * kick_dt, kick_dt, X);
         else
-            hipLaunchKernelGGL(k_half_finish<0>, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F, s.vel.as<double4>(),
-                               0.0);
+            hipLaunchKernelGGL(k_half_finish<0>, dim3(nb256), dim3(256), 0, s.stream, A, s.n, F, V, 0.0, 0.0, X);
     }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
+    if (kick_dt > 0.0 && advance) s.mirror_fresh = 0;  // local positions advanced
     if (half && s.sync_calls) check_fix_flag(s);
 }
 

@@ -143,6 +143,32 @@ __global__ void k_ghost_refresh(double4* __restrict__ pos, double4* __restrict__
     }
 }
 
+// One pass over locals and ghosts after the force pass advanced the local
+// positions: ghost = owner + image * L, and every binned atom's mirror
+// entry (inv >= 0) is rewritten.
+__global__ void k_step_refresh(double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
+                               int n, int ng, double L0, double L1, double L2, const int* __restrict__ inv,
+                               double4* __restrict__ spos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n + ng) return;
+    double4 q;
+    if (i < n) {
+        q = pos[i];
+    } else {
+        const int g = i - n;
+        int ix, iy, iz;
+        img_decode(g_code[g] & 0xff, ix, iy, iz);
+        const double4 p = pos[g_owner[g]];
+        q.x = p.x + L0 * (double)ix;
+        q.y = p.y + L1 * (double)iy;
+        q.z = p.z + L2 * (double)iz;
+        q.w = p.w;
+        pos[i] = q;
+    }
+    const int k = inv[i];
+    if (k >= 0) spos[k] = q;
+}
+
 // REF_LAG refresh (diagnostic): the reference's pack from the immediate
 // source (which may be a not-yet-refreshed ghost) then unpack.
 __global__ void k_ghost_pack_lag(const double4* __restrict__ pos, const int* __restrict__ g_src, const int* __restrict__ g_code,
@@ -236,6 +262,20 @@ void refresh_ghosts(Sim& s) {
                            s.spos.as<double4>());
     }
     TMD_HIP(hipGetLastError());
+}
+
+void step_refresh(Sim& s) {
+    if (has_error() || s.n + s.ng == 0) return;
+    if (!s.inv_valid || s.ghost_mode != GHOST_FRESH) {
+        refresh_ghosts(s);
+        return;
+    }
+    const int nt = 256;
+    hipLaunchKernelGGL(k_step_refresh, dim3(div_up(s.n + s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
+                       (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0], s.g.len[1],
+                       s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+    TMD_HIP(hipGetLastError());
+    s.mirror_fresh = 1;
 }
 
 // Grow every per-atom array (locals + ghosts) to hold `need` atoms,

@@ -23,24 +23,27 @@ __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict_
     spos[k] = pos[parts[k]];
 }
 
+template <bool HALF>
 __global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restrict__ spos, const double4* __restrict__ pos,
-                                                        const int* __restrict__ cell_of, const int* __restrict__ start,
-                                                        const int* __restrict__ parts, const int* __restrict__ work,
-                                                        const int* __restrict__ tile_start, NbArgs a,
-                                                        int* __restrict__ nb_count, nb_t* __restrict__ nb_list,
+                                                        const int* __restrict__ cell_of, const int* __restrict__ inv,
+                                                        const int* __restrict__ start, const int* __restrict__ parts,
+                                                        const int* __restrict__ work, const int* __restrict__ tile_start,
+                                                        NbArgs a, int* __restrict__ nb_count, nb_t* __restrict__ nb_list,
                                                         int* __restrict__ overflow) {
     extern __shared__ double dyn[];
+    int* cst = reinterpret_cast<int*>(reinterpret_cast<char*>(dyn) + (size_t)kStageBytes * a.wcap);
     const int tile = blockIdx.x;
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
     __shared__ TileWin W;
     tile_window(tile, start, a.g, W);
+    stage_row_starts(W, start, a.g, cst);
     const WinStage sp = win_stage(dyn, a.wcap);
-    const bool staged = stage_window(W, a.wcap, spos, parts, sp);
+    const bool staged = stage_window(W, a.wcap, spos, HALF ? parts : nullptr, sp);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = nlist_atom(i, w, cell_of[i], pos[i], start, a, W, staged, sp, spos, parts, nb_list);
+        const int cnt = nlist_atom<HALF>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, staged, sp, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(overflow, cnt);
     }
@@ -114,15 +117,19 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         s.nb_list.reserve(sizeof(nb_t) * (size_t)ntile64 * 64 * (size_t)s.nb_cap, false, s.stream);
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
-        const int wcap = window_cap(s);
+        const int rsb = row_starts_bytes(s.tiles);
+        const int wcap = std::min(s.max_window, (kLdsBudget - rsb) / kStageBytes);
         NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
-        const size_t lds = (size_t)wcap * kStageBytes;
-        if (s.n > 0)
-            hipLaunchKernelGGL(k_nlist, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(),
-                               (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(),
-                               (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(),
-                               (const int*)s.work.as<int>(), (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(),
-                               s.nb_list.as<nb_t>(), counters + 2);
+        const size_t lds = (size_t)wcap * kStageBytes + rsb;
+        if (s.n > 0) {
+#define TMD_NLIST(H)                                                                                                   \
+    hipLaunchKernelGGL(k_nlist<H>, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(), \
+                       (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(), (const int*)s.cell_inv.as<int>(), \
+                       (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), \
+                       (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(), s.nb_list.as<nb_t>(), counters + 2)
+            if (half) TMD_NLIST(true); else TMD_NLIST(false);
+#undef TMD_NLIST
+        }
         TMD_HIP(hipGetLastError());
         int over = 0;
         TMD_HIP(hipMemcpyAsync(&over, counters + 2, sizeof(int), hipMemcpyDeviceToHost, s.stream));

@@ -235,10 +235,17 @@ static double host_temperature(const Sim& s) {
 
 static void rebuild(Sim& s, int half, double cutoff) {
     s.ng = 0;  // exchange_particles on one rank (comm/comm.impala:392-396)
+    hipEvent_t ev = nullptr;
+    s.timer.begin(K_BORDERS, s.stream, &ev);
     enforce_pbc(s);
     build_ghosts(s);
+    s.timer.end(K_BORDERS, s.stream, ev);
+    s.timer.begin(K_BIN, s.stream, &ev);
     bin_particles(s);
+    s.timer.end(K_BIN, s.stream, ev);
+    s.timer.begin(K_NEIGHBOR, s.stream, &ev);
     build_neighbor_lists(s, half, cutoff);
+    s.timer.end(K_NEIGHBOR, s.stream, ev);
 }
 
 // ---------------------------------------------------------------------------
@@ -294,6 +301,9 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     s.ng = 0;
     s.nb_cap = neighborlist_capacity > 0 ? neighborlist_capacity : 1;
     s.nb_valid = 0;
+    s.windows_valid = 0;
+    s.inv_valid = 0;
+    s.mirror_fresh = 0;
     s.h_mass.assign(masses, masses + nparticles);
     s.h_pos.resize(3 * (size_t)nparticles);
     s.h_vel.resize(3 * (size_t)nparticles);
@@ -756,23 +766,42 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
     s.sync_calls = 0;
     if (!s.nb_valid || s.nb_half != hf) rebuild(s, hf, cutoff + skin);
     int done = 0;
-    for (int j = 0; j < steps && !has_error(); ++j) {
-        initial_integration(s, dt);
-        if ((j + 1) % reneigh_every == 0) {
-            rebuild(s, hf, cutoff + skin);
-        } else {
-            hipEvent_t ev = nullptr;
-            s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
-            refresh_ghosts(s);
-            s.timer.end(K_GHOST_SYNC, s.stream, ev);
+    if (!dem && s.ghost_mode == GHOST_FRESH) {
+        // LJ: the force pass also performs the final kick of its step and
+        // the first kick + drift of the next one, so a step is a ghost/mirror
+        // refresh (or a rebuild) and one force pass
+        if (steps > 0) initial_integration(s, dt);
+        for (int j = 0; j < steps && !has_error(); ++j) {
+            if ((j + 1) % reneigh_every == 0) {
+                rebuild(s, hf, cutoff + skin);
+            } else {
+                hipEvent_t ev = nullptr;
+                s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
+                step_refresh(s);
+                s.timer.end(K_GHOST_SYNC, s.stream, ev);
+            }
+            compute_lj(s, hf, cutoff, p0, p1, dt, j + 1 < steps);
+            ++done;
         }
-        if (dem) {
-            compute_dem(s, hf, cutoff, p0, p1, p2, p3);
-            final_integration(s, dt);
-        } else {
-            compute_lj(s, hf, cutoff, p0, p1, dt);  // final kick fused
+    } else {
+        for (int j = 0; j < steps && !has_error(); ++j) {
+            initial_integration(s, dt);
+            if ((j + 1) % reneigh_every == 0) {
+                rebuild(s, hf, cutoff + skin);
+            } else {
+                hipEvent_t ev = nullptr;
+                s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
+                refresh_ghosts(s);
+                s.timer.end(K_GHOST_SYNC, s.stream, ev);
+            }
+            if (dem) {
+                compute_dem(s, hf, cutoff, p0, p1, p2, p3);
+                final_integration(s, dt);
+            } else {
+                compute_lj(s, hf, cutoff, p0, p1, dt);  // final kick fused
+            }
+            ++done;
         }
-        ++done;
     }
     s.sync_calls = saved_sync;
     if (!has_error()) TMD_HIP(hipStreamSynchronize(s.stream));

@@ -157,6 +157,9 @@ void scan_exclusive(const int* in, int* out, int n, int* total_dev, int* tiles, 
 // borders / ghosts
 void build_ghosts(Sim& s);
 void refresh_ghosts(Sim& s);
+// fresh ghost positions and the cell-ordered mirror of locals and ghosts in
+// one pass (after positions were advanced inside the force pass)
+void step_refresh(Sim& s);
 
 // geometry (core/grid.impala:94-98): world AABB, rank AABB, cell spacing
 void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell_spacing);
@@ -173,9 +176,10 @@ void refresh_mirror(Sim& s);
 void build_neighbor_lists(Sim& s, int half, double cutoff);
 
 // forces
-// kick_dt > 0 (full lists): the final velocity-Verlet kick v += F dt/2 is
-// fused into the force kernel
-void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt = 0.0);
+// kick_dt > 0: the final velocity-Verlet kick v += F dt/2 is fused into the
+// force pass; with advance also the next step's kick and drift (positions
+// move; the mirror must be refreshed before the next force evaluation)
+void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt = 0.0, bool advance = false);
 void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu);
 void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs);
 

@@ -101,6 +101,7 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
 #pragma unroll 1
         for (int hb = 0; hb < 8; hb += 4) {
         const int k0 = gi * 8 + hb;
+        if (k0 >= cnt) break;  // this lane's entries are done
         double dx[4], dy[4], dz[4], rs[4];
         bool ok[4];
 #pragma unroll
@@ -206,25 +207,32 @@ struct NbArgs {
     TileGeom g;
 };
 
-// Verlet list of local atom i (work slot w, cell c of the tile's grid) from
-// the staged window: 9 stencil rows of 3 cells in the
-// reference's order, `rsq <= cut2` in FP64 without contraction.  Accepted
-// window offsets are packed 8 to a 16-byte group in registers and each full
-// group is written with one store (one list column per lane, so a wave's
-// stores fall in one 1 KiB stripe).  Returns the count; groups beyond the
-// capacity are counted, not stored.
-__device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi, const int* __restrict__ start,
-                                          const NbArgs& a, const TileWin& W, bool staged, const WinStage& sp,
-                                          const double4* __restrict__ spos, const int* __restrict__ parts,
-                                          nb_t* __restrict__ nb_list) {
+// Verlet list of local atom i (work slot w, cell c of the tile's grid,
+// cell-ordered index k_i) from the staged window: 9 stencil rows of 3 cells
+// in the reference's order, `rsq <= cut2` in FP64 without contraction.  Row
+// bounds come from the staged row starts (cst).  Full lists exclude i by its
+// window offset, so they read no atom index; half lists read the staged
+// index for the j < i / ghost filter.  Accepted window offsets are packed 8
+// to a 16-byte group in registers (selects, no branches) and each full group
+// is written with one store (one list column per lane).  Returns the count;
+// groups beyond the capacity are counted, not stored.
+template <bool HALF>
+__device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const double4 pi, const int* cst, const NbArgs& a,
+                                          const TileWin& W, bool staged, const WinStage& sp, const double4* __restrict__ spos,
+                                          const int* __restrict__ parts, nb_t* __restrict__ nb_list) {
     const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
+    const int wy = a.g.ty + 2;
+    const int xo = cx - W.cx0;  // (cx - 1) - (cx0 - 1)
+    const int per = a.g.tx + 3;
+    const int s_own = (cz - (W.cz0 - 1)) * wy + (cy - (W.cy0 - 1));
+    const int t_i = W.seg_off[s_own] + (k_i - W.seg_beg[s_own]);
     int4* const col = reinterpret_cast<int4*>(nb_list) + (size_t)(w >> 6) * (size_t)a.cap * 8 + (w & 63);
     const int ngroups = a.cap >> 3;
     unsigned long long lo = 0ull, hi = 0ull;
     int cnt = 0;
     for (int r = 0; r < 9; ++r) {
-        int t0, t1;
-        win_stencil_row(cx, cy, cz, r % 3 - 1, r / 3 - 1, start, a.g, W, t0, t1);
+        const int s = s_own + (r / 3 - 1) * wy + (r % 3 - 1);
+        const int t0 = cst[s * per + xo], t1 = cst[s * per + xo + 3];
         // 4 candidates per pass: loads and distances first (independent),
         // then in-order appends
         for (int tb = t0; tb < t1; tb += 4) {
@@ -232,38 +240,40 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, const double4 pi,
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int t = min(tb + u, t1 - 1);
-                int j;
                 double xj, yj, zj;
+                bool keep = (tb + u < t1) && (t != t_i);
                 if (staged) {
                     xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
-                    j = sp.j[t];
+                    if (HALF) {
+                        const int j = sp.j[t];
+                        keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                    }
                 } else {
                     const int kk = win_index(t, W);
-                    j = parts[kk];
                     const double4 p = spos[kk];
                     xj = p.x; yj = p.y; zj = p.z;
+                    if (HALF) {
+                        const int j = parts[kk];
+                        keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                    }
                 }
-                bool keep = (tb + u < t1) && (j != i);
-                if (a.half) keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
                 const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                 const double rsq = dx * dx + dy * dy + dz * dz;
                 ok[u] = keep && rsq <= a.cut2;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                if (ok[u]) {
-                    const int p = cnt & 7;
-                    const unsigned long long v = (unsigned long long)(unsigned)(tb + u) << (16 * (p & 3));
-                    if (p < 4) lo |= v;
-                    else hi |= v;
-                    ++cnt;
-                    if ((cnt & 7) == 0) {
-                        const int grp = (cnt >> 3) - 1;
-                        if (grp < ngroups)
-                            col[(size_t)grp * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
-                                                              (int)(unsigned)(hi >> 32));
-                        lo = hi = 0ull;
-                    }
+                const int p = cnt & 7;
+                const unsigned long long v = (unsigned long long)(unsigned)(tb + u) << (16 * (p & 3));
+                lo |= (ok[u] && p < 4) ? v : 0ull;
+                hi |= (ok[u] && p >= 4) ? v : 0ull;
+                cnt += ok[u] ? 1 : 0;
+                if (ok[u] && (cnt & 7) == 0) {
+                    const int grp = (cnt >> 3) - 1;
+                    if (grp < ngroups)
+                        col[(size_t)grp * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
+                                                          (int)(unsigned)(hi >> 32));
+                    lo = hi = 0ull;
                 }
             }
         }

@@ -26,7 +26,9 @@ constexpr int kTileThreads = 512;  // threads per tile block
 constexpr int kMaxTile = 8;        // cells per tile dimension, at most
 constexpr int kMaxSeg = (kMaxTile + 2) * (kMaxTile + 2);
 constexpr int kWinMax = 65535;     // window offsets are 16-bit
-constexpr int kLdsBudget = 80 * 1024;  // LDS per block: two blocks per 160 KiB CU
+// dynamic LDS per block: two blocks per 160 KiB CU, with 1 KiB left for each
+// block's static LDS (window rows, reductions)
+constexpr int kLdsBudget = 80 * 1024 - 1024;
 constexpr int kStageBytes = 28;       // LDS bytes per staged window atom: x, y, z 24, index 4
 constexpr int kLdsHalfBudget = 144 * 1024;  // half-list LJ: window records + reaction accumulators (one block per CU)
 
@@ -109,6 +111,30 @@ __device__ __forceinline__ void win_stencil_row(int cx, int cy, int cz, int oy, 
     const int kb = start[row + cx - 1], ke = start[row + cx + 2];
     t0 = W.seg_off[s] + (kb - W.seg_beg[s]);
     t1 = t0 + (ke - kb);
+}
+
+// Window offsets of the cell starts of every window row, for the list
+// build: cst[s * (tx + 3) + (x - (cx0 - 1))] for x in [cx0 - 1, xhi + 1].
+// A local atom's stencil row then needs no global load.  Whole block.
+// cst holds (ty + 2)(tz + 2)(tx + 3) ints (row_starts_bytes).
+__host__ __device__ inline int row_starts_bytes(const TileGeom& g) { return (g.ty + 2) * (g.tz + 2) * (g.tx + 3) * 4; }
+__device__ __forceinline__ void stage_row_starts(const TileWin& W, const int* __restrict__ start, TileGeom g, int* cst) {
+    const int wy = g.ty + 2;
+    const int per = g.tx + 3;
+    const int xlo = W.cx0 - 1;
+    const int xhi = min(W.cx0 + g.tx, g.nx - 1);
+    for (int e = threadIdx.x; e < W.nseg * per; e += blockDim.x) {
+        const int s = e / per, xo = e % per;
+        const int y = W.cy0 - 1 + s % wy, z = W.cz0 - 1 + s / wy;
+        const int x = xlo + xo;
+        int v = 0;
+        if (y >= 0 && y < g.ny && z >= 0 && z < g.nz && x <= xhi + 1) {
+            const int row = (z * g.ny + y) * g.nx;
+            v = W.seg_off[s] + (start[row + x] - W.seg_beg[s]);
+        }
+        cst[s * per + xo] = v;
+    }
+    __syncthreads();
 }
 
 // Stage the window's FP64 positions and, when sj != nullptr, atom indices
