@@ -56,12 +56,17 @@ def run_variant(tm, precision, half, steps, warmup, profile=False):
     if warmup:
         sim.run_lj(warmup, REN, DT, half, RC, SKIN)
     sim.barrier()
+    th0 = sim.thermo(RC)
     t0 = time.perf_counter()
     done = sim.run_lj(steps, REN, DT, half, RC, SKIN)
     sim.barrier()
     t1 = time.perf_counter()
     assert done == steps
-    out = {"seconds": t1 - t0, "atom_steps_per_s": sim.nlocal * steps / (t1 - t0)}
+    th1 = sim.thermo(RC)
+    e0, e1 = th0["KE"] + th0["PE"], th1["KE"] + th1["PE"]
+    out = {"seconds": t1 - t0, "atom_steps_per_s": sim.nlocal * steps / (t1 - t0),
+           "energy_per_atom_start": e0 / sim.nlocal, "energy_per_atom_end": e1 / sim.nlocal,
+           "energy_drift_rel": (e1 - e0) / abs(e0), "T_end": th1["T"], "PE_end": th1["PE"], "P_end": th1["P"]}
     if profile:
         sim.profile(True)
         sim.profile_reset()
@@ -106,6 +111,40 @@ def cpu_baseline(sample_steps=20):
     return {"value": n * sample_steps / (t1 - t0), "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle, FCC 80^3 ({n} atoms), {sample_steps} steps incl. 1 rebuild, "
                       f"{threads} OpenMP threads (list build and pair force threaded), {t1 - t0:.1f} s"}
+
+
+def c1_leg(tm, nc=32, steps=100, cpu=True):
+    """The reference's default run (main.cpp defaults): FCC 32^3 (131,072
+    atoms), FP64, full list, 100 steps, on the GPU and on the CPU oracle
+    (the checker) from the same initial state; the final thermo of both."""
+    sim = tm.Simulation.fcc(nc, precision="fp64")
+    sim.setup(False, RC, SKIN)
+    sim.lennard_jones(False, RC)
+    sim.barrier()
+    t0 = time.perf_counter()
+    sim.run_lj(steps, REN, DT, False, RC, SKIN)
+    sim.barrier()
+    t1 = time.perf_counter()
+    th = sim.thermo(RC)
+    n = sim.nlocal
+    sim.close()
+    out = {"atoms": n, "steps": steps, "gpu_seconds": t1 - t0, "gpu_atom_steps_per_s": n * steps / (t1 - t0),
+           "gpu_T": th["T"], "gpu_PE": th["PE"], "gpu_P": th["P"]}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        threads = min(os.cpu_count() or 1, 16)
+        s = po.lj_benchmark(nc, threads=threads)
+        c0 = time.perf_counter()
+        s.run_lj(steps, REN, DT)
+        c1 = time.perf_counter()
+        s.lj(False, RC)
+        tc = s.thermo()
+        s.close()
+        out.update({"cpu_seconds": c1 - c0, "cpu_atom_steps_per_s": n * steps / (c1 - c0), "cpu_threads": threads,
+                    "cpu_T": tc["T"], "cpu_PE": tc["PE"], "cpu_P": tc["P"],
+                    "gpu_over_cpu": (c1 - c0) / (t1 - t0)})
+    return out
 
 
 def adjoint_leg(tm, precision, nc=16, steps=200, ck=20, fd=False):
@@ -295,6 +334,10 @@ def main():
             adjoint[prec] = adjoint_leg(tm, prec, fd=not args.quick)
             print(f"# adjoint {prec}: {adjoint[prec]}", file=sys.stderr, flush=True)
 
+    c1 = None
+    if not args.quick:
+        c1 = c1_leg(tm, cpu=not args.no_cpu)
+        print(f"# c1 32^3 fp64: {c1}", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batch:
         batched = batched_leg(tm, sequential=not args.quick)
@@ -335,6 +378,8 @@ def main():
     }
     if adjoint:
         line["adjoint"] = adjoint
+    if c1:
+        line["c1_32cube_fp64"] = c1
     if batched:
         line["batched"] = batched
     if dem:
