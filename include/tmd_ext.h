@@ -44,8 +44,9 @@ void tmd_set_lj_shift(int on);
 /* Cells per tile of the list/force kernels (one block per tile; 0 0 0 =
  * automatic, about 480 atoms per tile).  Takes effect at the next binning. */
 void tmd_set_tile(int tx, int ty, int tz);
-/* Threads per tile block of the list-build and pair kernels (multiple of 64,
- * 64..512; default 512). */
+/* Threads per tile block of the list-build and pair kernels: a multiple of
+ * 64 in 64..1024, or 0 (default) for automatic: one lane per local atom of
+ * the fullest tile, 256..768, set at each binning. */
 void tmd_set_tile_threads(int threads);
 
 /* ---- thermo (deterministic device reductions) ------------------------------

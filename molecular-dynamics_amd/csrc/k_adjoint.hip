@@ -36,7 +36,7 @@ struct AdjCoef {
 };
 
 template <int PREC, int SHIFT>
-__global__ void __launch_bounds__(kTileThreads) k_lj_adjoint(const double4* __restrict__ pos, const double4* __restrict__ spos,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_adjoint(const double4* __restrict__ pos, const double4* __restrict__ spos,
                                                              const int* __restrict__ start, const int* __restrict__ parts,
                                                              const int* __restrict__ work, const int* __restrict__ tile_start,
                                                              const int* __restrict__ nbc, const nb_t* __restrict__ nbl,
@@ -62,6 +62,9 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_adjoint(const double4* __re
     const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
     const double inv_eps = 1.0 / a.eps, inv_sig = 1.0 / a.sigma;
     double g_eps = 0.0, g_sig = 0.0;
+    // STAGED as a compile-time constant: LDS reads, not generic-pointer loads
+    auto atoms = [&](auto staged_c) {
+    constexpr bool STAGED = decltype(staged_c)::value;
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
@@ -73,7 +76,7 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_adjoint(const double4* __re
             const int t = nbl[nb16_slot(w, k, a.cap)];
             int j;
             double xj, yj, zj;
-            if (staged) {
+            if (STAGED) {
                 j = sj[t];
                 xj = sx[t]; yj = sy[t]; zj = sz[t];
             } else {
@@ -143,6 +146,9 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_adjoint(const double4* __re
         xb.z += az;
         xbar[i] = xb;
     }
+    };
+    if (staged) atoms(std::true_type{});
+    else atoms(std::false_type{});
     const double be = block_sum_double_any(g_eps);
     const double bs = block_sum_double_any(g_sig);
     if (threadIdx.x == 0) {

@@ -435,7 +435,8 @@ __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restr
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = nlist_atom<false>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, staged, sp, spos, parts, nb_list);
+        const int cnt = staged ? nlist_atom<false, true>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list)
+                               : nlist_atom<false, false>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(&flags[FL_LIST], cnt);
     }
@@ -517,7 +518,9 @@ __global__ void __launch_bounds__(kTileThreads) k_b_energy(const RepDesc* __rest
     double pe = 0.0, vir = 0.0, np = 0.0;
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
-        lj_energy_atom<SHIFT>(pos[work[w]], w, nbc[w], nbl, a, 0, D.a0 + D.n, staged, sx, sy, sz, sj, spos, parts, W, pe, vir,
+        if (staged) lj_energy_atom<SHIFT, true>(pos[work[w]], w, nbc[w], nbl, a, 0, D.a0 + D.n, sx, sy, sz, sj, spos, parts, W, pe, vir,
+                              np);
+        else lj_energy_atom<SHIFT, false>(pos[work[w]], w, nbc[w], nbl, a, 0, D.a0 + D.n, sx, sy, sz, sj, spos, parts, W, pe, vir,
                               np);
     }
     const double bpe = block_sum_double_any(pe);

@@ -68,7 +68,7 @@ __global__ void k_bin_sort(const int* __restrict__ start, int ncells, int n, int
 // Tile work order (see tmd_window.hpp): one thread per tile counts the
 // tile's locals and the size of its window.
 __global__ void k_tile_stats(const int* __restrict__ start, const int* __restrict__ local_count, TileGeom g, int ntiles,
-                             int* __restrict__ tile_count, int* __restrict__ max_window) {
+                             int* __restrict__ tile_count, int* __restrict__ max_window, int* __restrict__ max_locals) {
     const int tile = blockIdx.x * blockDim.x + threadIdx.x;
     if (tile >= ntiles) return;
     const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
@@ -87,7 +87,10 @@ __global__ void k_tile_stats(const int* __restrict__ start, const int* __restric
                 const int row = (z * g.ny + y) * g.nx;
                 S += start[row + xhi + 1] - start[row + xlo];
             }
-    if (cnt > 0) atomicMax(max_window, S);
+    if (cnt > 0) {
+        atomicMax(max_window, S);
+        atomicMax(max_locals, cnt);
+    }
 }
 
 // Work order: one block per tile writes the tile's locals in (z, y, x)
@@ -184,9 +187,9 @@ void bin_particles(Sim& s) {
     s.work_start.reserve(sizeof(int) * ((size_t)ntiles + 1), false, s.stream);  // per-tile counts
     if (has_error()) return;
     int* counters = s.counters.as<int>();
-    TMD_HIP(hipMemsetAsync(counters + 6, 0, sizeof(int), s.stream));
+    TMD_HIP(hipMemsetAsync(counters + 6, 0, 2 * sizeof(int), s.stream));
     hipLaunchKernelGGL(k_tile_stats, dim3(div_up(ntiles, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(),
-                       (const int*)s.cell_count.as<int>(), tg, ntiles, s.work_start.as<int>(), counters + 6);
+                       (const int*)s.cell_count.as<int>(), tg, ntiles, s.work_start.as<int>(), counters + 6, counters + 7);
     exclusive_scan_int(s.work_start.as<int>(), s.tile_start.as<int>(), ntiles, s.tile_start.as<int>() + ntiles, s);
     hipLaunchKernelGGL(k_tile_fill_block, dim3(ntiles), dim3(256), 0, s.stream, (const int*)s.cell_start.as<int>(),
                        (const int*)s.cell_count.as<int>(), (const int*)s.tile_start.as<int>(), tg,

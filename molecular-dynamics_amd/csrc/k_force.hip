@@ -46,7 +46,7 @@ namespace tmd {
 // neighbours come from the cell-ordered mirror, which this kernel does not
 // touch).  The arithmetic is that of the separate integration kernels.
 template <int PREC, int SHIFT, int UNIT, int KICK>
-__global__ void __launch_bounds__(kTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
                                                          double4* __restrict__ vel, double hdt, double dt,
                                                          double4* __restrict__ pos_out) {
     extern __shared__ double dyn[];
@@ -101,7 +101,7 @@ __device__ __forceinline__ unsigned long long to_fix(double v) { return (unsigne
 // depend on the order in which lanes and blocks run.  A window too large
 // for LDS sends reactions straight to the global accumulators.
 template <int PREC, int SHIFT>
-__global__ void __launch_bounds__(kTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS, LJArgs a, int wcap_half,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS, LJArgs a, int wcap_half,
                                                          double4* __restrict__ own, unsigned long long* __restrict__ acc,
                                                          int* __restrict__ flag) {
     extern __shared__ double dyn[];
@@ -262,7 +262,7 @@ __global__ void k_fix_to_force(unsigned long long* __restrict__ acc, int n, doub
 // local-local pairs 1 and local-ghost pairs 1/2.  Per-block partial sums in
 // a fixed order, reduced by one block in a fixed order: deterministic.
 template <int SHIFT>
-__global__ void __launch_bounds__(kTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARAMS, LJArgs a, int half,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARAMS, LJArgs a, int half,
                                                            double* __restrict__ partials) {
     extern __shared__ double dyn[];
     const int tile = blockIdx.x;
@@ -285,7 +285,8 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARA
     const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
-        lj_energy_atom<SHIFT>(pos[work[w]], w, nbc[w], nbl, a, half, a.n, staged, sx, sy, sz, sj, spos, parts, W, pe, vir, np);
+        if (staged) lj_energy_atom<SHIFT, true>(pos[work[w]], w, nbc[w], nbl, a, half, a.n, sx, sy, sz, sj, spos, parts, W, pe, vir, np);
+        else lj_energy_atom<SHIFT, false>(pos[work[w]], w, nbc[w], nbl, a, half, a.n, sx, sy, sz, sj, spos, parts, W, pe, vir, np);
     }
     const double bpe = block_sum_double_any(pe);
     const double bvir = block_sum_double_any(vir);
@@ -344,7 +345,7 @@ __device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R r
 }
 
 template <int PREC, int HALF>
-__global__ void __launch_bounds__(kTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __restrict__ vel, DEMArgs a,
+__global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __restrict__ vel, DEMArgs a,
                                                      double4* __restrict__ force, unsigned long long* __restrict__ acc,
                                                      int* __restrict__ flag) {
     extern __shared__ double dyn[];
@@ -358,6 +359,9 @@ __global__ void __launch_bounds__(kTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, co
     double* sz = dyn + 2 * a.wcap;
     int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
     const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    // STAGED as a compile-time constant: LDS reads, not generic-pointer loads
+    auto atoms = [&](auto staged_c) {
+    constexpr bool STAGED = decltype(staged_c)::value;
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
     const int w = w0 + at;
     const int i = work[w];
@@ -370,7 +374,7 @@ __global__ void __launch_bounds__(kTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, co
         const int t = nbl[nb16_slot(w, k, a.cap)];
         int j;
         double xj, yj, zj;
-        if (staged) {
+        if (STAGED) {
             j = sj[t];
             xj = sx[t]; yj = sy[t]; zj = sz[t];
         } else {
@@ -411,6 +415,9 @@ __global__ void __launch_bounds__(kTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, co
         force[i] = make_double4(fx, fy, fz, 0.0);
     }
     }
+    };
+    if (staged) atoms(std::true_type{});
+    else atoms(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +495,7 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         unsigned long long* A = s.acc.as<unsigned long long>();
         // window records (32 B) + three 64-bit reaction accumulators per atom
         const int per = kStageBytes + 3 * (int)sizeof(unsigned long long);
-        const int wh = std::min(s.max_window, std::min(s.lds_limit, kLdsHalfBudget) / per) & ~1;
+        const int wh = std::min(s.max_window + 2, std::min(s.lds_limit, kLdsHalfBudget) / per) & ~1;  // + sentinel slot
         const size_t lds = (size_t)wh * per;
         double4* F = s.force.as<double4>();
 #define TMD_LJ_HALF(P, SH) \

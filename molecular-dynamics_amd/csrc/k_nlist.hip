@@ -24,7 +24,7 @@ __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict_
 }
 
 template <bool HALF>
-__global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restrict__ spos, const double4* __restrict__ pos,
+__global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __restrict__ spos, const double4* __restrict__ pos,
                                                         const int* __restrict__ cell_of, const int* __restrict__ inv,
                                                         const int* __restrict__ start, const int* __restrict__ parts,
                                                         const int* __restrict__ work, const int* __restrict__ tile_start,
@@ -43,25 +43,36 @@ __global__ void __launch_bounds__(kTileThreads) k_nlist(const double4* __restric
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = nlist_atom<HALF>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, staged, sp, spos, parts, nb_list);
+        const int cnt = staged ? nlist_atom<HALF, true>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list)
+                               : nlist_atom<HALF, false>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(overflow, cnt);
     }
 }
 
 // ---------------------------------------------------------------------------
-int window_cap(const Sim& s) { return std::min(s.max_window, kLdsBudget / kStageBytes); }
+// Threads per tile block: one lane per local of the fullest tile (no tile
+// takes a second pass), in whole wavefronts, at most kAutoTileThreads so that
+// two blocks of the tile kernels (up to 80 VGPRs: 6 waves per SIMD) share a
+// CU.  An explicit choice (tmd_set_tile_threads) overrides.
+int tile_block_threads(int requested, int max_locals) {
+    if (requested > 0) return requested;
+    const int t = (std::max(max_locals, 1) + kWave - 1) / kWave * kWave;
+    return std::min(std::max(t, kTileThreads / 2), kAutoTileThreads);
+}
+
+int window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kStageBytes); }  // + sentinel slot
 
 void build_windows(Sim& s) {
     if (has_error()) return;
     int* counters = s.counters.as<int>();
     const Geom& G = s.g;
     const int ntiles = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
-    // counters[6] = largest tile window (from k_tile_stats)
-    int host[3] = {0, 0, 0};
+    // counters[6] = largest tile window, counters[7] = most locals in a tile (from k_tile_stats)
+    int host[4] = {0, 0, 0, 0};
     TMD_HIP(hipMemcpyAsync(&host[0], s.tile_start.as<int>() + ntiles, sizeof(int), hipMemcpyDeviceToHost, s.stream));
     TMD_HIP(hipMemcpyAsync(&host[1], s.cell_start.as<int>() + G.ncells, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    TMD_HIP(hipMemcpyAsync(&host[2], counters + 6, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipMemcpyAsync(&host[2], counters + 6, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
     TMD_HIP(hipStreamSynchronize(s.stream));
     if (has_error()) return;
     if (host[0] != s.n) {
@@ -75,6 +86,7 @@ void build_windows(Sim& s) {
     }
     s.nbinned = host[1];
     s.max_window = host[2];
+    s.tile_threads = tile_block_threads(s.tile_threads_req, host[3]);
     s.spos.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
     if (has_error()) return;
     if (s.max_window > kWinMax) {
@@ -118,7 +130,7 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
         const int rsb = row_starts_bytes(s.tiles);
-        const int wcap = std::min(s.max_window, (kLdsBudget - rsb) / kStageBytes);
+        const int wcap = std::min(s.max_window + 1, (kLdsBudget - rsb) / kStageBytes);  // + sentinel slot
         NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
         const size_t lds = (size_t)wcap * kStageBytes + rsb;
         if (s.n > 0) {

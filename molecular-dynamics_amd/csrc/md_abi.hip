@@ -623,11 +623,12 @@ void tmd_set_lj_shift(int on) { sim().lj_shift = on ? 1 : 0; }
 
 void tmd_set_tile_threads(int threads) {
     Sim& s = sim();
-    if (threads < 64 || threads > kTileThreads || threads % 64 != 0) {
-        set_error(__FILE__, __LINE__, "tmd_set_tile_threads: threads must be a multiple of 64 in [64, 512]");
+    if (threads != 0 && (threads < 64 || threads > kMaxTileThreads || threads % 64 != 0)) {
+        set_error(__FILE__, __LINE__, "tmd_set_tile_threads: threads must be 0 (automatic) or a multiple of 64 in [64, 1024]");
         return;
     }
-    s.tile_threads = threads;
+    s.tile_threads_req = threads;
+    if (threads > 0) s.tile_threads = threads;
 }
 
 void tmd_set_tile(int tx, int ty, int tz) {

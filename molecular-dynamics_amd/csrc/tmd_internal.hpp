@@ -95,7 +95,8 @@ struct Sim {
     int max_window = 0; // largest tile window (atoms)
     TileGeom tiles{};   // tile decomposition of the cell grid
     int tile_req[3] = {0, 0, 0};  // requested cells per tile (0: automatic)
-    int tile_threads = kTileThreads;  // threads per tile block (multiple of 64, <= kTileThreads)
+    int tile_threads = kTileThreads;  // threads per tile block in use (multiple of 64, <= kMaxTileThreads)
+    int tile_threads_req = 0;         // requested block size, 0: automatic (tile_block_threads)
     int windows_valid = 0;
     int inv_valid = 0;      // cell_inv matches the current atom set (binning after the last ghost build)
     int mirror_fresh = 0;   // spos[k] == pos[cell_parts[k]] for every binned atom
@@ -171,7 +172,8 @@ TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz);
 
 // tile windows (after binning), position mirror, neighbour lists
 void build_windows(Sim& s);
-int window_cap(const Sim& s);  // staged window atoms per block (LDS budget, kStageBytes each)
+int window_cap(const Sim& s);
+int tile_block_threads(int requested, int max_locals);  // staged window atoms per block (LDS budget, kStageBytes each)
 void refresh_mirror(Sim& s);
 void build_neighbor_lists(Sim& s, int half, double cutoff);
 

@@ -49,7 +49,8 @@ template <int SHIFT, int UNIT>
 __device__ __forceinline__ float lj_f32u(const LJArgs& a, float rsq) {
     const float sr2 = __builtin_amdgcn_rcpf(rsq);
     const float sr6 = UNIT ? sr2 * sr2 * sr2 : sr2 * sr2 * sr2 * a.sigma6f;
-    float f = UNIT ? 48.0f * sr6 * (sr6 - 0.5f) * sr2 : 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
+    // UNIT: 48 sr6 (sr6 - 1/2) sr2 as (sr6 sr2) (48 sr6 - 24), one FMA
+    float f = UNIT ? (sr6 * sr2) * __builtin_fmaf(48.0f, sr6, -24.0f) : 48.0f * sr6 * (sr6 - 0.5f) * sr2 * a.epsf;
     if (SHIFT) f = f - a.fcf - (rsq - a.rc2f) * a.dfcf;
     return f;
 }
@@ -85,7 +86,10 @@ __device__ __forceinline__ void win_pos(int t, const WinStage& st, const double4
 // replaced by zero) and accumulated in list order.  Adding a masked term
 // adds dx * 0.0 = +-0.0, which leaves the running sum unchanged (the sum
 // starts at +0.0 and never becomes -0.0), so the FP64 result equals the
-// sequential reference sum bit for bit.
+// sequential reference sum bit for bit.  From a staged window, the entries
+// past the count in the last group are the sentinel offset (beyond the
+// cutoff), so only the cutoff test masks; from the mirror, the count does.
+// FP32 variants form rsq with FMAs; FP64 keeps the reference's rounding.
 template <int PREC, int SHIFT, int UNIT, bool STAGED>
 __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const WinStage& sp,
                                                 const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
@@ -106,13 +110,14 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
         bool ok[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const bool valid = k0 + u < cnt;
+            const bool valid = STAGED || k0 + u < cnt;
             double xj, yj, zj;
             win_pos<STAGED>(valid ? nb_entry(e, hb + u) : 0, sp, spos, W, xj, yj, zj);
             dx[u] = pi.x - xj;
             dy[u] = pi.y - yj;
             dz[u] = pi.z - zj;
-            rs[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            rs[u] = PREC == PREC_FP64 ? dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u]
+                                      : fma(dx[u], dx[u], fma(dy[u], dy[u], dz[u] * dz[u]));
             ok[u] = valid && rs[u] < a.rc2;
         }
         // The pair scalar is evaluated for every entry and masked afterwards:
@@ -162,9 +167,9 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
 // (sr12 - sr6) [+ shift terms], W = f r^2.  Full lists weight every entry
 // 1/2; half lists weight local-local entries 1 and local-ghost entries 1/2
 // (an index j >= nloc_end is a ghost).
-template <int SHIFT>
+template <int SHIFT, bool STAGED>
 __device__ __forceinline__ void lj_energy_atom(const double4 pi, int w, int cnt, const nb_t* __restrict__ nbl, const LJArgs& a,
-                                               int half, int nloc_end, bool staged, const double* sx, const double* sy,
+                                               int half, int nloc_end, const double* sx, const double* sy,
                                                const double* sz, const int* sj, const double4* __restrict__ spos,
                                                const int* __restrict__ parts, const TileWin& W, double& pe, double& vir,
                                                double& np) {
@@ -172,7 +177,7 @@ __device__ __forceinline__ void lj_energy_atom(const double4 pi, int w, int cnt,
         const int t = nbl[nb16_slot(w, k, a.cap)];
         int j;
         double xj, yj, zj;
-        if (staged) {
+        if (STAGED) {
             j = sj[t];
             xj = sx[t]; yj = sy[t]; zj = sz[t];
         } else {
@@ -210,15 +215,21 @@ struct NbArgs {
 // Verlet list of local atom i (work slot w, cell c of the tile's grid,
 // cell-ordered index k_i) from the staged window: 9 stencil rows of 3 cells
 // in the reference's order, `rsq <= cut2` in FP64 without contraction.  Row
-// bounds come from the staged row starts (cst).  Full lists exclude i by its
-// window offset, so they read no atom index; half lists read the staged
-// index for the j < i / ghost filter.  Accepted window offsets are packed 8
-// to a 16-byte group in registers (selects, no branches) and each full group
-// is written with one store (one list column per lane).  Returns the count;
-// groups beyond the capacity are counted, not stored.
-template <bool HALF>
+// bounds come from the staged row starts (cst).  Each row is tested in
+// chunks of 32 candidates that produce a bit mask (bit b: candidate tc + b
+// accepted); i itself is removed from the mask by its window offset, so full
+// lists read no atom index (half lists read the staged index for the j < i /
+// ghost filter).  The set bits are then appended in ascending order, which is
+// the reference's order, packed 8 to a 16-byte group in registers, and each
+// full group is written with one store (one list column per lane).  Only
+// accepted candidates pay for the packing.  The last group's free slots hold
+// the window's sentinel offset.  Returns the count; groups beyond the
+// capacity are counted, not stored.  STAGED is a template parameter so that
+// the staged loop addresses LDS directly (ds_read) rather than through
+// generic pointers.
+template <bool HALF, bool STAGED>
 __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const double4 pi, const int* cst, const NbArgs& a,
-                                          const TileWin& W, bool staged, const WinStage& sp, const double4* __restrict__ spos,
+                                          const TileWin& W, const WinStage& sp, const double4* __restrict__ spos,
                                           const int* __restrict__ parts, nb_t* __restrict__ nb_list) {
     const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
     const int wy = a.g.ty + 2;
@@ -233,42 +244,43 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
     for (int r = 0; r < 9; ++r) {
         const int s = s_own + (r / 3 - 1) * wy + (r % 3 - 1);
         const int t0 = cst[s * per + xo], t1 = cst[s * per + xo + 3];
-        // 4 candidates per pass: loads and distances first (independent),
-        // then in-order appends
-        for (int tb = t0; tb < t1; tb += 4) {
-            bool ok[4];
+        for (int tc = t0; tc < t1; tc += 32) {
+            const int te = min(tc + 32, t1);
+            unsigned m = 0u;
+            for (int tb = tc; tb < te; tb += 4) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = min(tb + u, t1 - 1);
-                double xj, yj, zj;
-                bool keep = (tb + u < t1) && (t != t_i);
-                if (staged) {
-                    xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
-                    if (HALF) {
-                        const int j = sp.j[t];
-                        keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                for (int u = 0; u < 4; ++u) {
+                    // past te, t repeats the chunk's last candidate (same bit, same outcome)
+                    const int t = min(tb + u, te - 1);
+                    double xj, yj, zj;
+                    int j = 0;
+                    if (STAGED) {
+                        xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
+                        if (HALF) j = sp.j[t];
+                    } else {
+                        const int kk = win_index(t, W);
+                        const double4 p = spos[kk];
+                        xj = p.x; yj = p.y; zj = p.z;
+                        if (HALF) j = parts[kk];
                     }
-                } else {
-                    const int kk = win_index(t, W);
-                    const double4 p = spos[kk];
-                    xj = p.x; yj = p.y; zj = p.z;
-                    if (HALF) {
-                        const int j = parts[kk];
-                        keep = keep && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
-                    }
+                    const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                    const double rsq = dx * dx + dy * dy + dz * dz;
+                    bool ok = rsq <= a.cut2;
+                    if (HALF) ok = ok && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                    m |= (ok ? 1u : 0u) << (t - tc);
                 }
-                const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-                const double rsq = dx * dx + dy * dy + dz * dz;
-                ok[u] = keep && rsq <= a.cut2;
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            const unsigned d = (unsigned)(t_i - tc);
+            m &= ~(d < 32u ? 1u << d : 0u);  // i is not its own neighbour
+            while (m != 0u) {
+                const int t = tc + __builtin_ctz(m);
+                m &= m - 1u;
                 const int p = cnt & 7;
-                const unsigned long long v = (unsigned long long)(unsigned)(tb + u) << (16 * (p & 3));
-                lo |= (ok[u] && p < 4) ? v : 0ull;
-                hi |= (ok[u] && p >= 4) ? v : 0ull;
-                cnt += ok[u] ? 1 : 0;
-                if (ok[u] && (cnt & 7) == 0) {
+                const unsigned long long v = (unsigned long long)(unsigned)t << (16 * (p & 3));
+                lo |= p < 4 ? v : 0ull;
+                hi |= p >= 4 ? v : 0ull;
+                ++cnt;
+                if ((cnt & 7) == 0) {
                     const int grp = (cnt >> 3) - 1;
                     if (grp < ngroups)
                         col[(size_t)grp * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
@@ -278,9 +290,16 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
             }
         }
     }
-    if ((cnt & 7) != 0 && (cnt >> 3) < ngroups)
+    if ((cnt & 7) != 0 && (cnt >> 3) < ngroups) {
+        // the last group's free slots hold the window's sentinel offset
+        const unsigned long long pad = (unsigned long long)(unsigned)win_size(W);
+        for (int p = cnt & 7; p < 8; ++p) {
+            if (p < 4) lo |= pad << (16 * p);
+            else hi |= pad << (16 * (p - 4));
+        }
         col[(size_t)(cnt >> 3) * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
                                                  (int)(unsigned)(hi >> 32));
+    }
     return cnt;
 }
 

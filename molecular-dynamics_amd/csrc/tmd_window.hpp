@@ -22,7 +22,9 @@
 
 namespace tmd {
 
-constexpr int kTileThreads = 512;  // threads per tile block
+constexpr int kTileThreads = 512;  // threads per tile block (default)
+constexpr int kMaxTileThreads = 1024;  // launch bound of the single-box tile kernels
+constexpr int kAutoTileThreads = 768;   // largest automatic block (two blocks per CU at <= 80 VGPRs)
 constexpr int kMaxTile = 8;        // cells per tile dimension, at most
 constexpr int kMaxSeg = (kMaxTile + 2) * (kMaxTile + 2);
 constexpr int kWinMax = 65535;     // window offsets are 16-bit
@@ -172,13 +174,19 @@ __device__ __forceinline__ WinStage win_stage(double* dyn, int wcap) {
     return st;
 }
 
-// Stage the window (positions, and atom indices when parts != nullptr) into
-// LDS if it fits wcap atoms.  Whole block.  Returns whether it was staged;
-// otherwise consumers read the mirror.
+// Position of the sentinel slot that follows a staged window (offset
+// win_size): list groups are padded with that offset, and its distance to
+// any atom of the box is far beyond every cutoff, so a padded entry fails
+// the cutoff test like any other out-of-range neighbour.
+constexpr double kFarPos = 1.0e18;
+
+// Stage the window (positions, and atom indices when parts != nullptr) and
+// the sentinel slot into LDS if they fit wcap atoms.  Whole block.  Returns
+// whether it was staged; otherwise consumers read the mirror.
 __device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const double4* __restrict__ spos,
                                              const int* __restrict__ parts, const WinStage& st) {
     const int S = win_size(W);
-    const bool staged = S <= wcap;
+    const bool staged = S < wcap;
     if (staged) {
         for (int t = threadIdx.x; t < S; t += blockDim.x) {
             const int k = win_index(t, W);
@@ -187,6 +195,12 @@ __device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const d
             st.xyz[3 * t + 1] = p.y;
             st.xyz[3 * t + 2] = p.z;
             if (parts) st.j[t] = parts[k];
+        }
+        if (threadIdx.x == 0) {
+            st.xyz[3 * S + 0] = kFarPos;
+            st.xyz[3 * S + 1] = kFarPos;
+            st.xyz[3 * S + 2] = kFarPos;
+            if (parts) st.j[S] = -1;
         }
     }
     __syncthreads();

@@ -94,8 +94,8 @@ class Simulation:
         self.L.tmd_set_precision(PRECISION[precision])
 
     def set_tile(self, tx=0, ty=None, tz=None, threads=None):
-        """Cells per tile (0: automatic) and threads per tile block; applies
-        from the next binning."""
+        """Cells per tile (0: automatic) and threads per tile block (0:
+        automatic); the tile applies from the next binning."""
         ty = tx if ty is None else ty
         tz = tx if tz is None else tz
         self.L.tmd_set_tile(tx, ty, tz)
