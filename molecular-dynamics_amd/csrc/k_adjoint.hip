@@ -206,7 +206,7 @@ static void snapshot_to(Sim& s, DevBuf& bx, DevBuf& bv, size_t slot) {
 }
 
 static void restore_from(Sim& s, const DevBuf& bx, const DevBuf& bv, size_t slot) {
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     copy_state(s, s.pos.as<double4>(), s.vel.as<double4>(), bx.as<double4>() + slot * s.n, bv.as<double4>() + slot * s.n);
 }
 

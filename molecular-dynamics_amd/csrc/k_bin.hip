@@ -99,7 +99,8 @@ __global__ void k_tile_stats(const int* __restrict__ start, const int* __restric
 // grouping lanes by list length).
 __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__ start, const int* __restrict__ local_count,
                                                          const int* __restrict__ tile_start, TileGeom g,
-                                                         const int* __restrict__ parts, int* __restrict__ work) {
+                                                         const int* __restrict__ parts, int* __restrict__ work,
+                                                         int* __restrict__ work_k) {
     __shared__ int coff[kMaxTile * kMaxTile * kMaxTile + 1];
     const int tile = blockIdx.x;
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
@@ -122,7 +123,10 @@ __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__
     for (int q = threadIdx.x; q < nc; q += blockDim.x) {
         const int c = ((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex);
         const int b = start[c];
-        for (int a = 0; a < coff[q + 1] - coff[q]; ++a) work[w0 + coff[q] + a] = parts[b + a];
+        for (int a = 0; a < coff[q + 1] - coff[q]; ++a) {
+            work[w0 + coff[q] + a] = parts[b + a];
+            work_k[w0 + coff[q] + a] = b + a;  // the atom's mirror index
+        }
     }
 }
 
@@ -163,9 +167,10 @@ void bin_particles(Sim& s) {
     s.cell_count.reserve(sizeof(int) * (size_t)G.ncells, false, s.stream);
     s.cell_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
     s.work.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
+    s.work_k.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
     if (has_error()) return;
     BinArgs a{G.aabb[0], G.aabb[2], G.aabb[4], G.spacing, G.nx, G.ny, G.ncells, nall};
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     s.inv_valid = 0;
     const int nt = 256;
     TMD_HIP(hipMemsetAsync(s.cell_count.p, 0, sizeof(int) * (size_t)G.ncells, s.stream));
@@ -193,7 +198,7 @@ void bin_particles(Sim& s) {
     exclusive_scan_int(s.work_start.as<int>(), s.tile_start.as<int>(), ntiles, s.tile_start.as<int>() + ntiles, s);
     hipLaunchKernelGGL(k_tile_fill_block, dim3(ntiles), dim3(256), 0, s.stream, (const int*)s.cell_start.as<int>(),
                        (const int*)s.cell_count.as<int>(), (const int*)s.tile_start.as<int>(), tg,
-                       (const int*)s.cell_parts.as<int>(), s.work.as<int>());
+                       (const int*)s.cell_parts.as<int>(), s.work.as<int>(), s.work_k.as<int>());
     TMD_HIP(hipGetLastError());
     s.windows_valid = 0;
     build_windows(s);

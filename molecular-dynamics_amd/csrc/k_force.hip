@@ -42,13 +42,17 @@ namespace tmd {
 // KICK 0: force only.  KICK 1: plus the final velocity-Verlet kick
 // v += F dt/2.  KICK 2: plus the next step's first kick and drift,
 // v += F dt/2, x += v dt, so the step loop needs no separate integration
-// pass (the lane's own position is read and written only by its lane; the
-// neighbours come from the cell-ordered mirror, which this kernel does not
-// touch).  The arithmetic is that of the separate integration kernels.
+// pass.  Positions (the lane's own and its neighbours') come from the
+// cell-ordered mirror, at the atom's mirror index wk[w] for its own.  KICK 2
+// writes the new position only into the second mirror (spos_next[wk[w]]),
+// not into pos, and no force: inside the fused step loop the force of a
+// step that is followed by another one is consumed here (the kick) and the
+// loop restores pos from the mirror before it rebuilds or returns.  The
+// arithmetic is that of the separate integration kernels.
 template <int PREC, int SHIFT, int UNIT, int KICK>
 __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
                                                          double4* __restrict__ vel, double hdt, double dt,
-                                                         double4* __restrict__ pos_out) {
+                                                         const int* __restrict__ wk, double4* __restrict__ spos_next) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -60,12 +64,13 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const double4 pi = pos[i];
+        const int kw = wk[w];
+        const double4 pi = spos[kw];
         const int cnt = nbc[w];
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
         const double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
                                  : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
-        force[i] = f;
+        if (KICK != 2) force[i] = f;
         if (KICK) {
             // velocity-Verlet final kick (k_final_integration's arithmetic)
             double4 v = vel[i];
@@ -81,7 +86,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
                 x.x = x.x + v.x * dt;
                 x.y = x.y + v.y * dt;
                 x.z = x.z + v.z * dt;
-                pos_out[i] = x;
+                spos_next[kw] = x;
             }
             vel[i] = v;
         }
@@ -464,18 +469,19 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
         const double hdt = 0.5 * kick_dt;
         double4* V = s.vel.as<double4>();
-        double4* X = s.pos.as<double4>();
+        const int* WK = s.work_k.as<int>();
+        double4* SN = s.spos2.as<double4>();
 #define TMD_LJ_FULL(P, SH, U)                                                                                          \
     do {                                                                                                               \
         if (kick_dt > 0.0 && advance)                                                                                  \
             hipLaunchKernelGGL((k_lj_full<P, SH, U, 2>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, X);                                                                            \
+                               kick_dt, WK, SN);                                                                       \
         else if (kick_dt > 0.0)                                                                                        \
             hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, X);                                                                            \
+                               kick_dt, WK, SN);                                                                       \
         else                                                                                                           \
             hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, X);                                                                            \
+                               kick_dt, WK, SN);                                                                       \
     } while (0)
         if (s.precision == PREC_FP64) {
             if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }
@@ -518,7 +524,18 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
     }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
-    if (kick_dt > 0.0 && advance) s.mirror_fresh = 0;  // local positions advanced
+    if (kick_dt > 0.0 && advance) {
+        if (!half) {
+            // the new local positions are in spos2: it becomes the mirror, whose
+            // ghost entries step_refresh rewrites
+            std::swap(s.spos, s.spos2);
+            s.mirror_fresh = 0;
+            s.ghost_stale = 1;
+            s.pos_stale = 1;
+        } else {
+            mirror_invalidate(s);  // local positions advanced
+        }
+    }
     if (half && s.sync_calls) check_fix_flag(s);
 }
 

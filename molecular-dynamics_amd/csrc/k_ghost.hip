@@ -169,6 +169,27 @@ __global__ void k_step_refresh(double4* __restrict__ pos, const int* __restrict_
     if (k >= 0) spos[k] = q;
 }
 
+// Ghost half of k_step_refresh, after a force pass that wrote the advanced
+// local positions into the mirror only: ghost = owner + image * L, with the
+// owner read from its mirror entry; written to pos and to the ghost's
+// mirror entry (owners are locals, so no entry is both read and written).
+__global__ void k_ghost_step(double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code, int n,
+                             int ng, double L0, double L1, double L2, const int* __restrict__ inv, double4* __restrict__ spos) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    int ix, iy, iz;
+    img_decode(g_code[g] & 0xff, ix, iy, iz);
+    const double4 p = spos[inv[g_owner[g]]];
+    double4 q;
+    q.x = p.x + L0 * (double)ix;
+    q.y = p.y + L1 * (double)iy;
+    q.z = p.z + L2 * (double)iz;
+    q.w = p.w;
+    pos[n + g] = q;
+    const int k = inv[n + g];
+    if (k >= 0) spos[k] = q;
+}
+
 // REF_LAG refresh (diagnostic): the reference's pack from the immediate
 // source (which may be a not-yet-refreshed ghost) then unpack.
 __global__ void k_ghost_pack_lag(const double4* __restrict__ pos, const int* __restrict__ g_src, const int* __restrict__ g_code,
@@ -205,7 +226,7 @@ void build_ghosts(Sim& s) {
     if (has_error()) return;
     s.ng = 0;
     s.inv_valid = 0;
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     const Geom& G = s.g;
     const double dspacing = G.spacing * 2.0;
     int* counters = s.counters.as<int>();
@@ -264,17 +285,46 @@ void refresh_ghosts(Sim& s) {
     TMD_HIP(hipGetLastError());
 }
 
+// pos[parts[k]] = mirror[k] for the locals of the cell CSR
+__global__ void k_unmirror(const double4* __restrict__ spos, const int* __restrict__ parts, int nbinned, int n,
+                           double4* __restrict__ pos) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbinned) return;
+    const int j = parts[k];
+    if (j < n) pos[j] = spos[k];
+}
+
+void sync_pos_from_mirror(Sim& s) {
+    if (has_error() || !s.pos_stale) return;
+    const int nt = 256;
+    if (s.nbinned > 0)
+        hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.spos.as<double4>(),
+                           (const int*)s.cell_parts.as<int>(), s.nbinned, s.n, s.pos.as<double4>());
+    TMD_HIP(hipGetLastError());
+    s.pos_stale = 0;
+}
+
 void step_refresh(Sim& s) {
     if (has_error() || s.n + s.ng == 0) return;
     if (!s.inv_valid || s.ghost_mode != GHOST_FRESH) {
+        s.ghost_stale = 0;
         refresh_ghosts(s);
         return;
     }
     const int nt = 256;
-    hipLaunchKernelGGL(k_step_refresh, dim3(div_up(s.n + s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
-                       (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0], s.g.len[1],
-                       s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+    if (s.ghost_stale) {
+        // locals are current in the mirror: rewrite the ghosts only
+        if (s.ng > 0)
+            hipLaunchKernelGGL(k_ghost_step, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
+                               (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0],
+                               s.g.len[1], s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+    } else {
+        hipLaunchKernelGGL(k_step_refresh, dim3(div_up(s.n + s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
+                           (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0], s.g.len[1],
+                           s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+    }
     TMD_HIP(hipGetLastError());
+    s.ghost_stale = 0;
     s.mirror_fresh = 1;
 }
 

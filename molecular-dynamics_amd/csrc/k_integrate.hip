@@ -81,7 +81,7 @@ void initial_integration(Sim& s, double dt) {
         hipLaunchKernelGGL(k_initial_integration<0>, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
                            s.vel.as<double4>(), (const double4*)s.force.as<double4>(), s.n, 0.5 * dt, dt, (const int*)nullptr,
                            (double4*)nullptr);
-    if (!s.inv_valid) s.mirror_fresh = 0;
+    if (!s.inv_valid) mirror_invalidate(s);
     s.timer.end(K_INTEGRATE_1, s.stream, ev);
     TMD_HIP(hipGetLastError());
 }
@@ -106,7 +106,7 @@ void enforce_pbc(Sim& s) {
         a.len[d] = s.g.len[d];
     }
     const int nt = 256;
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     hipLaunchKernelGGL(k_enforce_pbc, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.n, a);
     TMD_HIP(hipGetLastError());
 }

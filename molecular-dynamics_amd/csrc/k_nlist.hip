@@ -88,6 +88,7 @@ void build_windows(Sim& s) {
     s.max_window = host[2];
     s.tile_threads = tile_block_threads(s.tile_threads_req, host[3]);
     s.spos.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+    s.spos2.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
     if (has_error()) return;
     if (s.max_window > kWinMax) {
         char msg[200];

@@ -159,7 +159,7 @@ static void upload(Sim& s) {
         p[i] = make_double4(s.h_pos[3 * i], s.h_pos[3 * i + 1], s.h_pos[3 * i + 2], 0.0);
         v[i] = make_double4(s.h_vel[3 * i], s.h_vel[3 * i + 1], s.h_vel[3 * i + 2], 0.0);
     }
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     TMD_HIP(hipMemcpyAsync(s.pos.p, p.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.vel.p, v.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.mass.p, s.h_mass.data(), sizeof(double) * nall, hipMemcpyHostToDevice, s.stream));
@@ -303,7 +303,7 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     s.nb_valid = 0;
     s.windows_valid = 0;
     s.inv_valid = 0;
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     s.h_mass.assign(masses, masses + nparticles);
     s.h_pos.resize(3 * (size_t)nparticles);
     s.h_vel.resize(3 * (size_t)nparticles);
@@ -435,7 +435,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.tile_start, &s.spos,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.tile_start, &s.spos, &s.spos2,
                       &s.nb_count, &s.nb_list,
                       &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
@@ -446,7 +446,7 @@ void md_deallocate_grid() {
     s.nb_valid = 0;
     s.windows_valid = 0;
     s.inv_valid = 0;
-    s.mirror_fresh = 0;
+    mirror_invalidate(s);
     s.nbinned = 0;
     s.dev_valid = false;
 }
@@ -774,6 +774,7 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         if (steps > 0) initial_integration(s, dt);
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {
+                sync_pos_from_mirror(s);  // the rebuild reads pos
                 rebuild(s, hf, cutoff + skin);
             } else {
                 hipEvent_t ev = nullptr;
@@ -784,6 +785,7 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
             compute_lj(s, hf, cutoff, p0, p1, dt, j + 1 < steps);
             ++done;
         }
+        sync_pos_from_mirror(s);
     } else {
         for (int j = 0; j < steps && !has_error(); ++j) {
             initial_integration(s, dt);

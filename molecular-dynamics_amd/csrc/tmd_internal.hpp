@@ -100,6 +100,9 @@ struct Sim {
     int windows_valid = 0;
     int inv_valid = 0;      // cell_inv matches the current atom set (binning after the last ghost build)
     int mirror_fresh = 0;   // spos[k] == pos[cell_parts[k]] for every binned atom
+    int ghost_stale = 0;    // the last force pass advanced the locals in the mirror; the ghosts (pos and
+                            // mirror) still hold the previous step
+    int pos_stale = 0;      // local positions in pos lag the mirror (fused step loop; sync_pos_from_mirror)
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
@@ -123,6 +126,9 @@ struct Sim {
     DevBuf work_start, work;           // per-tile local counts; locals in tile work order (work slots)
     DevBuf tile_start;                 // first work slot of each tile [ntiles + 1]
     DevBuf spos;                       // cell-ordered position mirror (double4)
+    DevBuf spos2;                      // second mirror: the force pass that advances positions writes the
+                                       // new local positions here, then the two are swapped
+    DevBuf work_k;                     // mirror (CSR) index of each work slot's atom
     DevBuf nb_count, nb_list;
     DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
@@ -141,6 +147,12 @@ struct Sim {
     long long force_iters = 0, rsq_iters = 0, integration_iters = 0;
 };
 
+// Positions changed outside the fused step loop: the mirror is rebuilt from
+// pos before the next pair pass.
+inline void mirror_invalidate(Sim& s) {
+    s.mirror_fresh = 0;
+    s.ghost_stale = 0;
+}
 Sim& sim();
 bool sim_alive();
 
@@ -161,6 +173,7 @@ void refresh_ghosts(Sim& s);
 // fresh ghost positions and the cell-ordered mirror of locals and ghosts in
 // one pass (after positions were advanced inside the force pass)
 void step_refresh(Sim& s);
+void sync_pos_from_mirror(Sim& s);
 
 // geometry (core/grid.impala:94-98): world AABB, rank AABB, cell spacing
 void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell_spacing);
