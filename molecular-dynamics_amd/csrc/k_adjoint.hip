@@ -280,8 +280,7 @@ static void forward_traj(Sim& s, const TrajSpec& p, bool store_ckpt, double* uba
         initial_integration(s, p.dt);
         if (t % p.ck == 0) {
             s.ng = 0;
-            enforce_pbc(s);
-            build_ghosts(s);
+            build_ghosts(s, true);  // md_enforce_pbc + borders
             bin_particles(s);
             build_neighbor_lists(s, 0, cut);
         } else {

@@ -34,17 +34,64 @@ __device__ __forceinline__ void img_decode(int c, int& ix, int& iy, int& iz) {
 
 __device__ __forceinline__ double comp(const double4& p, int d) { return d == 0 ? p.x : (d == 1 ? p.y : p.z); }
 
-__global__ void __launch_bounds__(kScanThreads) k_border_count(const double4* __restrict__ pos, BorderArgs a,
+// Border selection bits of an atom, bit 2d: coordinate d above the upper
+// select threshold (the ghost of image -1 side), bit 2d + 1: below the lower
+// one.  A ghost made in pass d differs from its source only in coordinate d,
+// so it inherits the source's bits of the later passes.  The bits are
+// computed once, over the locals, together with the ghost totals of the
+// three passes: a local whose family (itself and its ghosts of earlier
+// passes) has m members contributes m to each later pass it is selected in.
+struct FaceArgs {
+    int n, wrap;
+    double hi_thr[3], lo_thr[3];
+    PbcArgs pbc;
+};
+
+__global__ void __launch_bounds__(kScanThreads) k_face_flags(double4* __restrict__ pos, FaceArgs a, unsigned char* __restrict__ face,
+                                                             int* __restrict__ totals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int t[6] = {0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        double4 p = pos[i];
+        if (a.wrap) {
+            pbc_wrap(p, a.pbc);  // md_enforce_pbc, fused
+            pos[i] = p;
+        }
+        const double c[3] = {p.x, p.y, p.z};
+        int bits = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            bits |= (c[d] > a.hi_thr[d] ? 1 : 0) << (2 * d);
+            bits |= (c[d] < a.lo_thr[d] ? 1 : 0) << (2 * d + 1);
+        }
+        face[i] = (unsigned char)bits;
+        const int fx = 1 + (bits & 1) + ((bits >> 1) & 1);       // the local and its x ghosts
+        const int fy = fx * (1 + ((bits >> 2) & 1) + ((bits >> 3) & 1));  // ... and their y ghosts
+        t[0] = bits & 1;
+        t[1] = (bits >> 1) & 1;
+        t[2] = ((bits >> 2) & 1) * fx;
+        t[3] = ((bits >> 3) & 1) * fx;
+        t[4] = ((bits >> 4) & 1) * fy;
+        t[5] = ((bits >> 5) & 1) * fy;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int v = block_sum_int(t[k]);
+        if (threadIdx.x == 0 && v) atomicAdd(&totals[k], v);
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_border_count(const unsigned char* __restrict__ face, BorderArgs a,
                                                                int* __restrict__ tiles0, int* __restrict__ tiles1) {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int c0 = 0, c1 = 0;
 #pragma unroll
     for (int it = 0; it < kScanItems; ++it) {
-        int i = base + it;
+        const int i = base + it;
         if (i < a.scan_end) {
-            double p = comp(pos[i], a.d);
-            c0 += (p > a.hi_thr) ? 1 : 0;
-            c1 += (p < a.lo_thr) ? 1 : 0;
+            const int b = face[i] >> (2 * a.d);
+            c0 += b & 1;
+            c1 += (b >> 1) & 1;
         }
     }
     int t0 = block_sum_int(c0);
@@ -57,7 +104,8 @@ __global__ void __launch_bounds__(kScanThreads) k_border_count(const double4* __
 
 __device__ __forceinline__ void write_ghost(int i, int slot, int side, const BorderArgs& a, double4* __restrict__ pos,
                                             double4* __restrict__ vel, double* __restrict__ mass,
-                                            int* __restrict__ g_owner, int* __restrict__ g_src, int* __restrict__ g_code) {
+                                            int* __restrict__ g_owner, int* __restrict__ g_src, int* __restrict__ g_code,
+                                            unsigned char* __restrict__ face) {
     const double4 p = pos[i];
     const double pd = comp(p, a.d);
     int fl = (side == 0) ? (pd > a.whi_thr ? -1 : 0) : (pd < a.wlo_thr ? 1 : 0);
@@ -85,13 +133,15 @@ __device__ __forceinline__ void write_ghost(int i, int slot, int side, const Bor
     g_owner[g] = owner;
     g_src[g] = i;
     g_code[g] = img_encode(ix, iy, iz) | (img_encode(f[0], f[1], f[2]) << 8);
+    face[slot] = face[i];
 }
 
 __global__ void __launch_bounds__(kScanThreads) k_border_scatter(double4* __restrict__ pos, double4* __restrict__ vel,
                                                                  double* __restrict__ mass, int* __restrict__ g_owner,
                                                                  int* __restrict__ g_src, int* __restrict__ g_code,
                                                                  const int* __restrict__ off0, const int* __restrict__ off1,
-                                                                 const int* __restrict__ totals, BorderArgs a) {
+                                                                 const int* __restrict__ totals, BorderArgs a,
+                                                                 unsigned char* __restrict__ face) {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int f0[kScanItems], f1[kScanItems];
     int c0 = 0, c1 = 0;
@@ -100,9 +150,9 @@ __global__ void __launch_bounds__(kScanThreads) k_border_scatter(double4* __rest
         int i = base + it;
         f0[it] = f1[it] = 0;
         if (i < a.scan_end) {
-            double p = comp(pos[i], a.d);
-            f0[it] = (p > a.hi_thr) ? 1 : 0;
-            f1[it] = (p < a.lo_thr) ? 1 : 0;
+            const int b = face[i] >> (2 * a.d);
+            f0[it] = b & 1;
+            f1[it] = (b >> 1) & 1;
         }
         c0 += f0[it];
         c1 += f1[it];
@@ -114,8 +164,8 @@ __global__ void __launch_bounds__(kScanThreads) k_border_scatter(double4* __rest
 #pragma unroll
     for (int it = 0; it < kScanItems; ++it) {
         int i = base + it;
-        if (f0[it]) { write_ghost(i, a.base + r0, 0, a, pos, vel, mass, g_owner, g_src, g_code); r0++; }
-        if (f1[it]) { write_ghost(i, a.base + total0 + r1, 1, a, pos, vel, mass, g_owner, g_src, g_code); r1++; }
+        if (f0[it]) { write_ghost(i, a.base + r0, 0, a, pos, vel, mass, g_owner, g_src, g_code, face); r0++; }
+        if (f1[it]) { write_ghost(i, a.base + total0 + r1, 1, a, pos, vel, mass, g_owner, g_src, g_code, face); r1++; }
     }
 }
 
@@ -222,46 +272,73 @@ __global__ void k_ghost_unpack_lag(double4* __restrict__ pos, double4* __restric
 
 static void ensure_atom_capacity(Sim& s, int need);
 
-void build_ghosts(Sim& s) {
+// One pass over the locals (optionally wrapping them into the box first)
+// gives every atom's selection bits and the ghost totals of the three passes,
+// read back with one synchronisation; the passes then run back to back on
+// the stream: per pass a count over the bits, a tile scan and the stable
+// scatter, in the reference's order.
+void build_ghosts(Sim& s, bool wrap) {
     if (has_error()) return;
     s.ng = 0;
     s.inv_valid = 0;
     mirror_invalidate(s);
+    if (s.n == 0) return;
     const Geom& G = s.g;
     const double dspacing = G.spacing * 2.0;
     int* counters = s.counters.as<int>();
+    s.face.reserve((size_t)std::max(s.cap_all, s.n), true, s.stream);
+    if (has_error()) return;
+    FaceArgs fa;
+    fa.n = s.n;
+    fa.wrap = wrap ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+        fa.hi_thr[d] = G.aabb[2 * d + 1] - dspacing;
+        fa.lo_thr[d] = G.aabb[2 * d] + dspacing;
+    }
+    fa.pbc = pbc_args(s);
+    TMD_HIP(hipMemsetAsync(counters + 8, 0, 6 * sizeof(int), s.stream));
+    hipLaunchKernelGGL(k_face_flags, dim3(div_up(s.n, kScanThreads)), dim3(kScanThreads), 0, s.stream, s.pos.as<double4>(), fa,
+                       s.face.as<unsigned char>(), counters + 8);
+    int tot[6] = {0, 0, 0, 0, 0, 0};
+    TMD_HIP(hipMemcpyAsync(tot, counters + 8, sizeof tot, hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipStreamSynchronize(s.stream));
+    if (has_error()) return;
+    const int ng_all = tot[0] + tot[1] + tot[2] + tot[3] + tot[4] + tot[5];
+    ensure_atom_capacity(s, s.n + ng_all);
+    if (has_error()) return;
+    const int max_tiles = div_up(s.n + ng_all, kScanTile);
+    s.scan_blocks.reserve(sizeof(int) * 2 * (size_t)max_tiles, false, s.stream);
+    if (has_error()) return;
+    int ng = 0;
     for (int d = 0; d < 3; ++d) {
         BorderArgs a;
         a.d = d;
         a.n = s.n;
-        a.scan_end = s.n + s.ng;
-        a.base = s.n + s.ng;
-        a.hi_thr = G.aabb[2 * d + 1] - dspacing;
-        a.lo_thr = G.aabb[2 * d] + dspacing;
+        a.scan_end = s.n + ng;
+        a.base = s.n + ng;
+        a.hi_thr = fa.hi_thr[d];
+        a.lo_thr = fa.lo_thr[d];
         a.whi_thr = G.world[2 * d + 1] - G.spacing;
         a.wlo_thr = G.world[2 * d] + G.spacing;
         a.len0 = G.len[0]; a.len1 = G.len[1]; a.len2 = G.len[2];
-        const int ntiles = div_up(a.scan_end, kScanTile);
-        s.scan_blocks.reserve(sizeof(int) * 2 * (size_t)ntiles, false, s.stream);
-        int* t0 = s.scan_blocks.as<int>();
-        int* t1 = t0 + ntiles;
-        hipLaunchKernelGGL(k_border_count, dim3(ntiles), dim3(kScanThreads), 0, s.stream, (const double4*)s.pos.as<double4>(), a, t0, t1);
-        scan_tiles(s, t0, ntiles, counters + 0);
-        scan_tiles(s, t1, ntiles, counters + 1);
-        int tot[2] = {0, 0};
-        TMD_HIP(hipMemcpyAsync(tot, counters, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
-        TMD_HIP(hipStreamSynchronize(s.stream));
-        if (has_error()) return;
-        const int add = tot[0] + tot[1];
-        ensure_atom_capacity(s, s.n + s.ng + add);
-        if (has_error()) return;
-        // scan_blocks may not be reallocated by ensure_atom_capacity (separate buffer)
-        hipLaunchKernelGGL(k_border_scatter, dim3(ntiles), dim3(kScanThreads), 0, s.stream, s.pos.as<double4>(), s.vel.as<double4>(),
-                           s.mass.as<double>(), s.g_owner.as<int>(), s.g_src.as<int>(), s.g_code.as<int>(), (const int*)t0,
-                           (const int*)t1, (const int*)counters, a);
-        TMD_HIP(hipGetLastError());
-        s.ng += add;
+        const int add = tot[2 * d] + tot[2 * d + 1];
+        if (add > 0) {
+            const int ntiles = div_up(a.scan_end, kScanTile);
+            int* t0 = s.scan_blocks.as<int>();
+            int* t1 = t0 + ntiles;
+            hipLaunchKernelGGL(k_border_count, dim3(ntiles), dim3(kScanThreads), 0, s.stream,
+                               (const unsigned char*)s.face.as<unsigned char>(), a, t0, t1);
+            scan_tiles(s, t0, ntiles, counters + 0);
+            scan_tiles(s, t1, ntiles, counters + 1);
+            hipLaunchKernelGGL(k_border_scatter, dim3(ntiles), dim3(kScanThreads), 0, s.stream, s.pos.as<double4>(),
+                               s.vel.as<double4>(), s.mass.as<double>(), s.g_owner.as<int>(), s.g_src.as<int>(),
+                               s.g_code.as<int>(), (const int*)t0, (const int*)t1, (const int*)counters, a,
+                               s.face.as<unsigned char>());
+            TMD_HIP(hipGetLastError());
+        }
+        ng += add;
     }
+    s.ng = ng;
 }
 
 void refresh_ghosts(Sim& s) {
@@ -343,6 +420,7 @@ static void ensure_atom_capacity(Sim& s, int need) {
     s.cell_slot.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.cell_parts.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.cell_inv.reserve(sizeof(int) * (size_t)cap, false, s.stream);
+    s.face.reserve((size_t)cap, true, s.stream);
     s.inv_valid = 0;
     s.cap_all = cap;
 }

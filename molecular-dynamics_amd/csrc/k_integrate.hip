@@ -41,18 +41,11 @@ __global__ void k_final_integration(double4* __restrict__ vel, const double4* __
     vel[i] = v;
 }
 
-struct PbcArgs {
-    double lo[3], hi[3], len[3];
-};
-
 __global__ void k_enforce_pbc(double4* __restrict__ pos, int n, PbcArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
-    double q;
-    q = p.x; if (p.x < a.lo[0]) q += a.len[0]; if (p.x > a.hi[0]) q -= a.len[0]; p.x = q;
-    q = p.y; if (p.y < a.lo[1]) q += a.len[1]; if (p.y > a.hi[1]) q -= a.len[1]; p.y = q;
-    q = p.z; if (p.z < a.lo[2]) q += a.len[2]; if (p.z > a.hi[2]) q -= a.len[2]; p.z = q;
+    pbc_wrap(p, a);
     pos[i] = p;
 }
 
@@ -97,14 +90,19 @@ void final_integration(Sim& s, double dt) {
     TMD_HIP(hipGetLastError());
 }
 
-void enforce_pbc(Sim& s) {
-    if (has_error() || s.n == 0) return;
+PbcArgs pbc_args(const Sim& s) {
     PbcArgs a;
     for (int d = 0; d < 3; ++d) {
         a.lo[d] = s.g.world[2 * d];
         a.hi[d] = s.g.world[2 * d + 1];
         a.len[d] = s.g.len[d];
     }
+    return a;
+}
+
+void enforce_pbc(Sim& s) {
+    if (has_error() || s.n == 0) return;
+    const PbcArgs a = pbc_args(s);
     const int nt = 256;
     mirror_invalidate(s);
     hipLaunchKernelGGL(k_enforce_pbc, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.n, a);

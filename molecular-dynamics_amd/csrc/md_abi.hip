@@ -237,8 +237,7 @@ static void rebuild(Sim& s, int half, double cutoff) {
     s.ng = 0;  // exchange_particles on one rank (comm/comm.impala:392-396)
     hipEvent_t ev = nullptr;
     s.timer.begin(K_BORDERS, s.stream, &ev);
-    enforce_pbc(s);
-    build_ghosts(s);
+    build_ghosts(s, true);  // md_enforce_pbc + borders
     s.timer.end(K_BORDERS, s.stream, ev);
     s.timer.begin(K_BIN, s.stream, &ev);
     bin_particles(s);
@@ -435,7 +434,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.tile_start, &s.spos, &s.spos2,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.tile_start, &s.spos, &s.spos2, &s.face,
                       &s.nb_count, &s.nb_list,
                       &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,

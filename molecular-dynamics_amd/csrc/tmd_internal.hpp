@@ -129,6 +129,7 @@ struct Sim {
     DevBuf spos2;                      // second mirror: the force pass that advances positions writes the
                                        // new local positions here, then the two are swapped
     DevBuf work_k;                     // mirror (CSR) index of each work slot's atom
+    DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
     DevBuf nb_count, nb_list;
     DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
@@ -168,7 +169,7 @@ void scan_tiles(Sim& s, int* tiles, int ntiles, int* total_dev);  // one-block e
 void scan_exclusive(const int* in, int* out, int n, int* total_dev, int* tiles, hipStream_t st);
 
 // borders / ghosts
-void build_ghosts(Sim& s);
+void build_ghosts(Sim& s, bool wrap = false);  // wrap: md_enforce_pbc on the locals first (fused)
 void refresh_ghosts(Sim& s);
 // fresh ghost positions and the cell-ordered mirror of locals and ghosts in
 // one pass (after positions were advanced inside the force pass)
@@ -206,6 +207,19 @@ void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part)
 void initial_integration(Sim& s, double dt);
 void final_integration(Sim& s, double dt);
 void enforce_pbc(Sim& s);
+
+// Periodic wrap of a local position into the box (md_enforce_pbc): one
+// conditional shift per coordinate.
+struct PbcArgs {
+    double lo[3], hi[3], len[3];
+};
+PbcArgs pbc_args(const Sim& s);
+__host__ __device__ inline void pbc_wrap(double4& p, const PbcArgs& a) {
+    double q;
+    q = p.x; if (p.x < a.lo[0]) q += a.len[0]; if (p.x > a.hi[0]) q -= a.len[0]; p.x = q;
+    q = p.y; if (p.y < a.lo[1]) q += a.len[1]; if (p.y > a.hi[1]) q -= a.len[1]; p.y = q;
+    q = p.z; if (p.z < a.lo[2]) q += a.len[2]; if (p.z > a.hi[2]) q -= a.len[2]; p.z = q;
+}
 double kinetic_sum(Sim& s);  // sum m v.v over locals (deterministic)
 int launch_kinetic(Sim& s, double* part);  // per-block partials of sum v.v; returns the block count
 
