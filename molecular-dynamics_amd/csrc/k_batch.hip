@@ -287,19 +287,24 @@ __global__ void k_b_bin_count(const RepDesc* __restrict__ R, const int* __restri
     const int r = cb_rep[blockIdx.x];
     const RepDesc& D = R[r];
     const int i = (blockIdx.x - D.cb0) * kBlk + threadIdx.x;
-    if (i >= D.n + dyn[r * DY_STRIDE + DY_NG]) return;
-    const double4 p = pos[D.a0 + i];
-    const double fx = floor((p.x - D.aabb[0]) / D.spacing);
-    const double fy = floor((p.y - D.aabb[2]) / D.spacing);
-    const double fz = floor((p.z - D.aabb[4]) / D.spacing);
+    const bool live = i < D.n + dyn[r * DY_STRIDE + DY_NG];
     int c = -1;
-    if (fabs(fx) < 1e9 && fabs(fy) < 1e9 && fabs(fz) < 1e9) {
-        const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
-        const long long flat = ((long long)ck * D.g.ny + cj) * D.g.nx + ci;
-        if (flat >= 0 && flat < D.ncells) c = (int)flat;
+    if (live) {
+        const double4 p = pos[D.a0 + i];
+        const double fx = floor((p.x - D.aabb[0]) / D.spacing);
+        const double fy = floor((p.y - D.aabb[2]) / D.spacing);
+        const double fz = floor((p.z - D.aabb[4]) / D.spacing);
+        if (fabs(fx) < 1e9 && fabs(fy) < 1e9 && fabs(fz) < 1e9) {
+            const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
+            const long long flat = ((long long)ck * D.g.ny + cj) * D.g.nx + ci;
+            if (flat >= 0 && flat < D.ncells) c = (int)flat;
+        }
     }
+    // all lanes take part; a block belongs to one replica, so cells do not mix
+    const int sl = wave_cell_slot(c >= 0 ? D.c0 + c : -1, count);
+    if (!live) return;
     cell_of[D.a0 + i] = c;
-    slot[D.a0 + i] = (c >= 0) ? atomicAdd(&count[D.c0 + c], 1) : -1;
+    slot[D.a0 + i] = sl;
     inv[D.a0 + i] = -1;  // set by k_b_bin_sort for binned atoms
 }
 
@@ -316,23 +321,11 @@ __global__ void k_b_bin_scatter(const RepDesc* __restrict__ R, const int* __rest
 
 __global__ void k_b_bin_sort(const RepDesc* __restrict__ R, const int* __restrict__ cell_rep, const int* __restrict__ start,
                              int ncells_all, int* __restrict__ parts, int* __restrict__ local_count, int* __restrict__ inv) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.x * (blockDim.x / kWave) + (int)threadIdx.x / kWave;  // one wavefront per cell
     if (c >= ncells_all) return;
     const int lim = R[cell_rep[c]].a0 + R[cell_rep[c]].n;
-    const int b = start[c], e = start[c + 1];
-    for (int k = b + 1; k < e; ++k) {
-        const int v = parts[k];
-        int m = k - 1;
-        while (m >= b && parts[m] > v) {
-            parts[m + 1] = parts[m];
-            --m;
-        }
-        parts[m + 1] = v;
-    }
-    int nl = 0;
-    while (b + nl < e && parts[b + nl] < lim) ++nl;
-    local_count[c] = nl;
-    for (int k = b; k < e; ++k) inv[parts[k]] = k;
+    const int nl = wave_sort_cell(start[c], start[c + 1], lim, parts, inv);
+    if ((threadIdx.x & (kWave - 1)) == 0) local_count[c] = nl;
 }
 
 __global__ void k_b_tile_stats(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep, const int* __restrict__ start,
@@ -793,7 +786,7 @@ static void enq_rebuild(Batch& B) {
     hipLaunchKernelGGL(k_b_bin_scatter, dim3(B.n_cb), dim3(kBlk), 0, st, R, (const int*)B.cb_rep.as<int>(),
                        (const int*)B.dyn.as<int>(), (const int*)B.cell_of.as<int>(), (const int*)B.cell_slot.as<int>(),
                        (const int*)B.cell_start.as<int>(), B.parts.as<int>());
-    hipLaunchKernelGGL(k_b_bin_sort, dim3(div_up(B.n_cells, kBlk)), dim3(kBlk), 0, st, R, (const int*)B.cell_rep.as<int>(),
+    hipLaunchKernelGGL(k_b_bin_sort, dim3(div_up(B.n_cells, kBlk / kWave)), dim3(kBlk), 0, st, R, (const int*)B.cell_rep.as<int>(),
                        (const int*)B.cell_start.as<int>(), B.n_cells, B.parts.as<int>(), B.cell_count.as<int>(),
                        B.cell_inv.as<int>());
     hipLaunchKernelGGL(k_b_tile_stats, dim3(div_up(B.n_tiles, kBlk)), dim3(kBlk), 0, st, R, (const int*)B.tile_rep.as<int>(),

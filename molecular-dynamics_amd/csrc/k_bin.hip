@@ -16,21 +16,24 @@ struct BinArgs {
 
 __global__ void k_bin_count(const double4* __restrict__ pos, BinArgs a, int* __restrict__ cell_of, int* __restrict__ slot,
                             int* __restrict__ count, int* __restrict__ inv) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nall) return;
-    const double4 p = pos[i];
-    const double fx = floor((p.x - a.xmin) / a.spacing);
-    const double fy = floor((p.y - a.ymin) / a.spacing);
-    const double fz = floor((p.z - a.zmin) / a.spacing);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = -1;
-    // non-finite or absurd coordinates are treated as outside the grid
-    if (fabs(fx) < 1e9 && fabs(fy) < 1e9 && fabs(fz) < 1e9) {
-        const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
-        const long long flat = ((long long)ck * a.ny + cj) * a.nx + ci;
-        if (flat >= 0 && flat < a.ncells) c = (int)flat;
+    if (i < a.nall) {
+        const double4 p = pos[i];
+        const double fx = floor((p.x - a.xmin) / a.spacing);
+        const double fy = floor((p.y - a.ymin) / a.spacing);
+        const double fz = floor((p.z - a.zmin) / a.spacing);
+        // non-finite or absurd coordinates are treated as outside the grid
+        if (fabs(fx) < 1e9 && fabs(fy) < 1e9 && fabs(fz) < 1e9) {
+            const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
+            const long long flat = ((long long)ck * a.ny + cj) * a.nx + ci;
+            if (flat >= 0 && flat < a.ncells) c = (int)flat;
+        }
     }
+    const int sl = wave_cell_slot(c, count);  // all lanes take part
+    if (i >= a.nall) return;
     cell_of[i] = c;
-    slot[i] = (c >= 0) ? atomicAdd(&count[c], 1) : -1;
+    slot[i] = sl;
     inv[i] = -1;  // set by k_bin_sort for binned atoms
 }
 
@@ -42,27 +45,15 @@ __global__ void k_bin_scatter(const int* __restrict__ cell_of, const int* __rest
     if (c >= 0) parts[start[c] + slot[i]] = i;
 }
 
-// One thread per cell: insertion sort of the (short) cell segment, then the
+// One wavefront per cell: sort the cell segment by atom index, then the
 // number of locals in it (locals have the lowest indices, so they lead every
 // sorted segment).
-__global__ void k_bin_sort(const int* __restrict__ start, int ncells, int n, int* __restrict__ parts,
-                           int* __restrict__ local_count, int* __restrict__ inv) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const int b = start[c], e = start[c + 1];
-    for (int k = b + 1; k < e; ++k) {
-        const int v = parts[k];
-        int m = k - 1;
-        while (m >= b && parts[m] > v) {
-            parts[m + 1] = parts[m];
-            --m;
-        }
-        parts[m + 1] = v;
-    }
-    int nl = 0;
-    while (b + nl < e && parts[b + nl] < n) ++nl;
-    local_count[c] = nl;
-    for (int k = b; k < e; ++k) inv[parts[k]] = k;
+__global__ void __launch_bounds__(256) k_bin_sort(const int* __restrict__ start, int ncells, int n, int* __restrict__ parts,
+                                                  int* __restrict__ local_count, int* __restrict__ inv) {
+    const int c = blockIdx.x * (blockDim.x / kWave) + (int)threadIdx.x / kWave;
+    if (c >= ncells) return;  // whole wavefronts
+    const int nl = wave_sort_cell(start[c], start[c + 1], n, parts, inv);
+    if ((threadIdx.x & (kWave - 1)) == 0) local_count[c] = nl;
 }
 
 // Tile work order (see tmd_window.hpp): one thread per tile counts the
@@ -181,7 +172,7 @@ void bin_particles(Sim& s) {
     if (nall > 0)
         hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(nall, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_of.as<int>(),
                            (const int*)s.cell_slot.as<int>(), (const int*)s.cell_start.as<int>(), nall, s.cell_parts.as<int>());
-    hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(), G.ncells,
+    hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt / kWave)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(), G.ncells,
                        s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(), s.cell_inv.as<int>());
     s.inv_valid = 1;
     // tile work order

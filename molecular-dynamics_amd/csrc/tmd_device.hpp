@@ -85,6 +85,62 @@ __device__ __forceinline__ double block_sum_double_any(double v) {
     return r;
 }
 
+// Counting-sort slot of a binned atom: lanes of one wavefront that fall in
+// the same cell take consecutive slots in lane order with one atomic per
+// (wavefront, cell).  c < 0: not binned (slot -1).  Every lane of the
+// wavefront must call it.
+__device__ __forceinline__ int wave_cell_slot(int c, int* __restrict__ count) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long pending = __ballot(c >= 0);
+    int slot = -1;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int cl = __shfl(c, leader, kWave);
+        const unsigned long long grp = __ballot(c == cl) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[cl], __popcll(grp));
+        base = __shfl(base, leader, kWave);
+        if ((grp >> lane) & 1ull) slot = base + __popcll(grp & ((1ull << lane) - 1ull));
+        pending &= ~grp;
+    }
+    return slot;
+}
+
+// Sort the cell segment parts[b, e) (distinct atom indices) ascending with
+// one wavefront and set inv[value] = its position.  Segments of up to 64 by
+// rank (each lane counts the smaller values); longer ones by lane 0
+// (insertion sort).  Returns, in every lane, how many values are below lim
+// (the locals, which lead the sorted segment).
+__device__ __forceinline__ int wave_sort_cell(int b, int e, int lim, int* __restrict__ parts, int* __restrict__ inv) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int m = e - b;
+    if (m <= kWave) {
+        const int v = lane < m ? parts[b + lane] : 0x7fffffff;
+        int rank = 0;
+        for (int k = 0; k < m; ++k) rank += (__shfl(v, k, kWave) < v) ? 1 : 0;
+        if (lane < m) {
+            parts[b + rank] = v;
+            inv[v] = b + rank;
+        }
+        return __popcll(__ballot(lane < m && v < lim));
+    }
+    int nl = 0;
+    if (lane == 0) {
+        for (int k = b + 1; k < e; ++k) {
+            const int v = parts[k];
+            int q = k - 1;
+            while (q >= b && parts[q] > v) {
+                parts[q + 1] = parts[q];
+                --q;
+            }
+            parts[q + 1] = v;
+        }
+        while (b + nl < e && parts[b + nl] < lim) ++nl;
+        for (int k = b; k < e; ++k) inv[parts[k]] = k;
+    }
+    return __shfl(nl, 0, kWave);
+}
+
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
 // with a private L2.  Remap so that XCD x processes one contiguous range of
 // logical blocks: atoms that are neighbours in cell order (and so share
