@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restr
     tile_window(tile - D.t0, st, D.g, W);
     stage_row_starts(W, st, D.g, cst);
     const WinStage sp = win_stage(dyn, a.wcap);
-    const bool staged = stage_window(W, a.wcap, spos, nullptr, sp);
+    const bool staged = stage_window(W, a.wcap, spos, nullptr, sp, kListPad);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __rest
     tile_window(tile, start, a.g, W);
     stage_row_starts(W, start, a.g, cst);
     const WinStage sp = win_stage(dyn, a.wcap);
-    const bool staged = stage_window(W, a.wcap, spos, HALF ? parts : nullptr, sp);
+    const bool staged = stage_window(W, a.wcap, spos, HALF ? parts : nullptr, sp, kListPad);
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
@@ -131,7 +131,7 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
         const int rsb = row_starts_bytes(s.tiles);
-        const int wcap = std::min(s.max_window + 1, (kLdsBudget - rsb) / kStageBytes);  // + sentinel slot
+        const int wcap = std::min(s.max_window + kListPad, (kLdsBudget - rsb) / kStageBytes);  // + sentinel slots
         NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
         const size_t lds = (size_t)wcap * kStageBytes + rsb;
         if (s.n > 0) {

@@ -139,8 +139,16 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
             // rounding)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
-                const double fm = (double)(ok[u] ? f : 0.0f);
+                double fm;
+                if (SHIFT) {
+                    const float f = lj_f32u<SHIFT, UNIT>(a, (float)rs[u]);
+                    fm = (double)(ok[u] ? f : 0.0f);
+                } else {
+                    // a masked entry gets rsq = +inf, so sr2 = 1/inf = 0 and
+                    // the plain-cutoff scalar is a signed zero (one 32-bit
+                    // select instead of a 64-bit one)
+                    fm = (double)lj_f32u<SHIFT, UNIT>(a, ok[u] ? (float)rs[u] : __builtin_inff());
+                }
                 fx = fma(dx[u], fm, fx);
                 fy = fma(dy[u], fm, fy);
                 fz = fma(dz[u], fm, fz);
@@ -248,10 +256,13 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
             const int te = min(tc + 32, t1);
             unsigned m = 0u;
             for (int tb = tc; tb < te; tb += 4) {
+                const unsigned b0 = 1u << (tb - tc);  // bit of candidate tb
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    // past te, t repeats the chunk's last candidate (same bit, same outcome)
-                    const int t = min(tb + u, te - 1);
+                    // staged: up to three candidates past te are read (window
+                    // atoms or sentinel slots) and masked below; from the
+                    // mirror, t is clamped to stay inside the window
+                    const int t = STAGED ? tb + u : min(tb + u, te - 1);
                     double xj, yj, zj;
                     int j = 0;
                     if (STAGED) {
@@ -267,9 +278,10 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
                     const double rsq = dx * dx + dy * dy + dz * dz;
                     bool ok = rsq <= a.cut2;
                     if (HALF) ok = ok && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
-                    m |= (ok ? 1u : 0u) << (t - tc);
+                    m |= ok ? b0 << u : 0u;
                 }
             }
+            m &= te - tc == 32 ? 0xffffffffu : (1u << (te - tc)) - 1u;  // candidates of this chunk only
             const unsigned d = (unsigned)(t_i - tc);
             m &= ~(d < 32u ? 1u << d : 0u);  // i is not its own neighbour
             while (m != 0u) {

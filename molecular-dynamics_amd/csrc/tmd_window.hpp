@@ -179,14 +179,18 @@ __device__ __forceinline__ WinStage win_stage(double* dyn, int wcap) {
 // any atom of the box is far beyond every cutoff, so a padded entry fails
 // the cutoff test like any other out-of-range neighbour.
 constexpr double kFarPos = 1.0e18;
+// sentinel slots of a list-build window: candidates are read four at a time
+// without clamping, up to three past the window end
+constexpr int kListPad = 3;
 
 // Stage the window (positions, and atom indices when parts != nullptr) and
-// the sentinel slot into LDS if they fit wcap atoms.  Whole block.  Returns
-// whether it was staged; otherwise consumers read the mirror.
+// `pad` sentinel slots (offsets S .. S + pad - 1) into LDS if they fit wcap
+// atoms.  Whole block.  Returns whether it was staged; otherwise consumers
+// read the mirror.
 __device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const double4* __restrict__ spos,
-                                             const int* __restrict__ parts, const WinStage& st) {
+                                             const int* __restrict__ parts, const WinStage& st, int pad = 1) {
     const int S = win_size(W);
-    const bool staged = S < wcap;
+    const bool staged = S + pad <= wcap;
     if (staged) {
         for (int t = threadIdx.x; t < S; t += blockDim.x) {
             const int k = win_index(t, W);
@@ -196,11 +200,12 @@ __device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const d
             st.xyz[3 * t + 2] = p.z;
             if (parts) st.j[t] = parts[k];
         }
-        if (threadIdx.x == 0) {
-            st.xyz[3 * S + 0] = kFarPos;
-            st.xyz[3 * S + 1] = kFarPos;
-            st.xyz[3 * S + 2] = kFarPos;
-            if (parts) st.j[S] = -1;
+        if ((int)threadIdx.x < pad) {
+            const int t = S + (int)threadIdx.x;
+            st.xyz[3 * t + 0] = kFarPos;
+            st.xyz[3 * t + 1] = kFarPos;
+            st.xyz[3 * t + 2] = kFarPos;
+            if (parts) st.j[t] = -1;
         }
     }
     __syncthreads();
