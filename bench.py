@@ -35,15 +35,17 @@ RC, SKIN, DT, REN = 2.5, 0.3, 0.005, 20
 
 def force_kernel_bytes(sim, precision):
     """Algorithmic HBM bytes of one step-loop pair-force launch (full list,
-    final kick fused): neighbour counts and work indices (4 B each per
-    atom), 4 B per list entry (the reference's i32 list; this build stores
-    2 B), each position read once (24 B, x/y/z FP64), each force written
-    once (24 B), each velocity read and written once by the kick (48 B)."""
+    kick and drift fused): neighbour count, atom index and mirror index of
+    each work slot (4 B each per atom), 4 B per list entry (the reference's
+    i32 list; this build stores 2 B), each position read once (24 B, x/y/z
+    FP64), each local's new position written once (24 B; the force of a step
+    that is followed by another stays in registers), each velocity read and
+    written once by the kicks (48 B)."""
     counts, _ = sim.neighbor_lists(with_lists=False)
     n = sim.nlocal
     nall = n + sim.nghost
     entries = int(counts.astype(np.int64).sum())
-    return 4 * n + 4 * n + 4 * entries + 24 * nall + 24 * n + 48 * n, entries
+    return 3 * 4 * n + 4 * entries + 24 * nall + 24 * n + 48 * n, entries
 
 
 def run_variant(tm, precision, half, steps, warmup, profile=False):
