@@ -141,14 +141,16 @@ int tmd_batch_create(int nrep, const int* counts, const struct Vector3D* pos, co
 int tmd_batch_setup(int handle, double cutoff, double skin, double epsilon, double sigma, double dt, int reneigh_every,
                     int precision, int lj_shift);
 /* Launch configuration (restarts from the initial state): cells per tile
- * edge (0 automatic per replica), threads per tile block (64..512, multiple
- * of 64), cell edge / (cutoff + skin) (>= 1), list capacity headroom over
- * the largest count, ghost room headroom over the expected ghost count. */
+ * edge (0 automatic per replica), threads per tile block (0: automatic, one
+ * lane per local of the fullest tile; else 64..1024, multiple of 64), cell
+ * edge / (cutoff + skin) (>= 1), list capacity headroom over the largest
+ * count, ghost room headroom over the expected ghost count. */
 int tmd_batch_configure(int handle, int tile, int threads, double spacing_factor, double list_headroom,
                         double ghost_headroom);
 /* Times trial runs of the candidate configurations from the initial state,
  * keeps the fastest and restarts from the initial state with it.  out
- * (may be NULL): tile, threads, spacing factor, list headroom, ms/step. */
+ * (may be NULL): tile, threads in use, spacing factor, list headroom,
+ * ms/step. */
 int tmd_batch_autotune(int handle, int trial_steps, double* out5);
 /* Restart from the initial state with the current configuration. */
 int tmd_batch_reset(int handle);

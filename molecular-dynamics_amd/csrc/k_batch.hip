@@ -45,6 +45,7 @@ struct RepDesc {
     int sb0, nsb;                // scan blocks (kScanTile slots) over [0, cap)
     int eb0, neb;                // 256-blocks over the locals
     int cb0, ncb;                // 256-blocks over [0, cap)
+    int gb0, ngb;                // 256-blocks over the ghost room [n, cap)
     TileGeom g;
     double aabb[6], len[3], spacing;
     double lo_pbc[3], hi_pbc[3];     // world bounds (periodic wrap)
@@ -55,7 +56,7 @@ struct RepDesc {
 // per-replica device counters
 enum { DY_NG = 0, DY_BASE = 1, DY_TOT0 = 4, DY_STRIDE = 8 };
 // device flags (max-reduced)
-enum { FL_GHOST = 0, FL_LIST = 1, FL_WIN = 2, FL_UNBINNED = 3, FL_MAXWIN = 4, FL_N = 8 };
+enum { FL_GHOST = 0, FL_LIST = 1, FL_WIN = 2, FL_UNBINNED = 3, FL_MAXWIN = 4, FL_MAXLOC = 5, FL_N = 8 };
 
 __device__ __forceinline__ int b_img_encode(int ix, int iy, int iz) { return (ix + 1) + 3 * (iy + 1) + 9 * (iz + 1); }
 __device__ __forceinline__ void b_img_decode(int c, int& ix, int& iy, int& iz) {
@@ -251,31 +252,39 @@ __global__ void __launch_bounds__(kScanThreads) k_b_border_scatter(const RepDesc
     }
 }
 
-// after a force pass that advanced the locals: fresh ghosts and the mirror
-// of every binned local and ghost
-__global__ void k_b_step_refresh(const RepDesc* __restrict__ R, const int* __restrict__ cb_rep, const int* __restrict__ dyn,
-                                 double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
-                                 const int* __restrict__ inv, double4* __restrict__ spos) {
-    const int r = cb_rep[blockIdx.x];
+// after a force pass that wrote the advanced locals into the mirror only:
+// ghost = owner + image * L with the owner read from its mirror entry, into
+// pos and the ghost's mirror entry (k_ghost_step's arithmetic)
+__global__ void k_b_ghost_step(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep, const int* __restrict__ dyn,
+                               double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
+                               const int* __restrict__ inv, double4* __restrict__ spos) {
+    const int r = gb_rep[blockIdx.x];
     const RepDesc& D = R[r];
-    const int i = (blockIdx.x - D.cb0) * kBlk + threadIdx.x;
-    if (i >= D.n + dyn[r * DY_STRIDE + DY_NG]) return;
-    const int s = D.a0 + i;
+    const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
+    if (g >= dyn[r * DY_STRIDE + DY_NG]) return;
+    const int s = D.a0 + D.n + g;
+    int ix, iy, iz;
+    b_img_decode(g_code[s], ix, iy, iz);
+    const double4 p = spos[inv[D.a0 + g_owner[s]]];
     double4 q;
-    if (i < D.n) {
-        q = pos[s];
-    } else {
-        int ix, iy, iz;
-        b_img_decode(g_code[s], ix, iy, iz);
-        const double4 p = pos[D.a0 + g_owner[s]];
-        q.x = p.x + D.len[0] * (double)ix;
-        q.y = p.y + D.len[1] * (double)iy;
-        q.z = p.z + D.len[2] * (double)iz;
-        q.w = p.w;
-        pos[s] = q;
-    }
+    q.x = p.x + D.len[0] * (double)ix;
+    q.y = p.y + D.len[1] * (double)iy;
+    q.z = p.z + D.len[2] * (double)iz;
+    q.w = p.w;
+    pos[s] = q;
     const int k = inv[s];
     if (k >= 0) spos[k] = q;
+}
+
+// pos of every local from its mirror entry (before a rebuild and at the end
+// of a run)
+__global__ void k_b_unmirror(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, const int* __restrict__ inv,
+                             const double4* __restrict__ spos, double4* __restrict__ pos) {
+    const int r = eb_rep[blockIdx.x];
+    const int i = (blockIdx.x - R[r].eb0) * kBlk + threadIdx.x;
+    if (i >= R[r].n) return;
+    const int s = R[r].a0 + i;
+    pos[s] = spos[inv[s]];
 }
 
 // ---------------------------------------------------------------------------
@@ -356,15 +365,21 @@ __global__ void k_b_tile_stats(const RepDesc* __restrict__ R, const int* __restr
             }
     if (cnt > 0) {
         atomicMax(&flags[FL_MAXWIN], S);
+        atomicMax(&flags[FL_MAXLOC], cnt);
         if (S > kWinMax) atomicMax(&flags[FL_WIN], S);
     }
 }
 
-__global__ void k_b_tile_fill(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep, const int* __restrict__ start,
-                              const int* __restrict__ local_count, const int* __restrict__ tile_start, int ntiles_all,
-                              const int* __restrict__ parts, int* __restrict__ work) {
-    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= ntiles_all) return;
+// one block per tile: the tile's locals in (z, y, x) cell order, and the
+// CSR (mirror) index of each (k_tile_fill_block's order)
+__global__ void __launch_bounds__(64) k_b_tile_fill(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+                                                    const int* __restrict__ start, const int* __restrict__ local_count,
+                                                    const int* __restrict__ tile_start, const int* __restrict__ parts,
+                                                    int* __restrict__ work, int* __restrict__ work_k) {
+    __shared__ int coff[kMaxTile * kMaxTile * kMaxTile + 1];
+    const int tile = blockIdx.x;
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
+    if (nloc == 0) return;
     const RepDesc& D = R[tile_rep[tile]];
     const TileGeom g = D.g;
     const int* st = start + D.c0;
@@ -373,14 +388,26 @@ __global__ void k_b_tile_fill(const RepDesc* __restrict__ R, const int* __restri
     const int tx_i = lt % g.ntx, ty_i = (lt / g.ntx) % g.nty, tz_i = lt / (g.ntx * g.nty);
     const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
     const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
-    int w = tile_start[tile];
-    for (int z = cz0; z < cz1; ++z)
-        for (int y = cy0; y < cy1; ++y)
-            for (int x = cx0; x < cx1; ++x) {
-                const int c = (z * g.ny + y) * g.nx + x;
-                const int b = st[c], nl = lc[c];
-                for (int a = 0; a < nl; ++a) work[w++] = parts[b + a];
-            }
+    const int ex = cx1 - cx0, ey = cy1 - cy0, ez = cz1 - cz0;
+    const int nc = ex * ey * ez;
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int q = 0; q < nc; ++q) {
+            const int c = ((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex);
+            coff[q] = o;
+            o += lc[c];
+        }
+        coff[nc] = o;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+        const int c = ((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex);
+        const int b = st[c];
+        for (int a = 0; a < coff[q + 1] - coff[q]; ++a) {
+            work[w0 + coff[q] + a] = parts[b + a];
+            work_k[w0 + coff[q] + a] = b + a;
+        }
+    }
 }
 
 // every replica's locals must all sit in its tiles
@@ -402,7 +429,7 @@ __global__ void k_b_mirror(const double4* __restrict__ pos, const int* __restric
 
 // ---------------------------------------------------------------------------
 // neighbour lists and forces: one block per tile of any replica
-__global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+__global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                           const double4* __restrict__ spos, const double4* __restrict__ pos,
                                                           const int* __restrict__ cell_of, const int* __restrict__ inv,
                                                           const int* __restrict__ start,
@@ -435,15 +462,17 @@ __global__ void __launch_bounds__(kTileThreads) k_b_nlist(const RepDesc* __restr
     }
 }
 
-// ADV 1: also the next step's first kick and drift (as k_lj_full<..., 2>)
+// Positions come from the mirror (own atom at wk[w]).  ADV 1: also the next
+// step's first kick and drift, written only into the second mirror, with no
+// force store (as k_lj_full<..., 2>); ADV 0 stores the force.
 template <int PREC, int SHIFT, int UNIT, int ADV>
-__global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
-                                                            const double4* __restrict__ pos, const double4* __restrict__ spos,
-                                                            const int* __restrict__ start, const int* __restrict__ work,
-                                                            const int* __restrict__ tile_start, const int* __restrict__ nbc,
-                                                            const nb_t* __restrict__ nbl, LJArgs a, double4* __restrict__ force,
-                                                            double4* __restrict__ vel, double hdt, double dt,
-                                                            double4* __restrict__ pos_out) {
+__global__ void __launch_bounds__(kMaxTileThreads) k_b_lj_full(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+                                                               const double4* __restrict__ spos, const int* __restrict__ start,
+                                                               const int* __restrict__ work, const int* __restrict__ wk,
+                                                               const int* __restrict__ tile_start, const int* __restrict__ nbc,
+                                                               const nb_t* __restrict__ nbl, LJArgs a,
+                                                               double4* __restrict__ force, double4* __restrict__ vel, double hdt,
+                                                               double dt, double4* __restrict__ spos_next) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -456,12 +485,13 @@ __global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __res
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const double4 pi = pos[i];
+        const int kw = wk[w];
+        const double4 pi = spos[kw];
         const int cnt = nbc[w];
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
         const double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
                                  : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
-        force[i] = f;
+        if (!ADV) force[i] = f;
         // velocity-Verlet final kick (k_final_integration's arithmetic)
         double4 v = vel[i];
         v.x = v.x + f.x * hdt;
@@ -476,14 +506,14 @@ __global__ void __launch_bounds__(kTileThreads) k_b_lj_full(const RepDesc* __res
             x.x = x.x + v.x * dt;
             x.y = x.y + v.y * dt;
             x.z = x.z + v.z * dt;
-            pos_out[i] = x;
+            spos_next[kw] = x;
         }
         vel[i] = v;
     }
 }
 
 template <int SHIFT>
-__global__ void __launch_bounds__(kTileThreads) k_b_energy(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+__global__ void __launch_bounds__(kMaxTileThreads) k_b_energy(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                            const double4* __restrict__ pos, const double4* __restrict__ spos,
                                                            const int* __restrict__ start, const int* __restrict__ parts,
                                                            const int* __restrict__ work, const int* __restrict__ tile_start,
@@ -559,7 +589,7 @@ __global__ void __launch_bounds__(kBlk) k_b_thermo_reduce(const RepDesc* __restr
 // host side
 struct BatchCfg {
     int tile = 0;               // cells per tile edge (0: per-replica automatic)
-    int threads = kTileThreads; // threads per tile block (multiple of 64, <= 512)
+    int threads = 0;            // threads per tile block (multiple of 64, <= 1024); 0: automatic
     double spacing = 1.0;       // cell edge / (cutoff + skin)
     double list_headroom = 1.2; // list capacity / largest count at set-up
     double ghost_headroom = 1.5;// ghost room / expected ghost count
@@ -576,11 +606,12 @@ struct Batch {
     double ghost_scale = 1.0;        // grows after a ghost overflow
 
     std::vector<RepDesc> R;
-    int n_sb = 0, n_eb = 0, n_cb = 0, n_tiles = 0, n_cells = 0, n_slots = 0, n_work = 0;
+    int n_sb = 0, n_eb = 0, n_cb = 0, n_gb = 0, n_tiles = 0, n_cells = 0, n_slots = 0, n_work = 0;
     int list_cap = 8, wcap = 0;
-    DevBuf dR, sb_rep, eb_rep, cb_rep, tile_rep, cell_rep;
-    DevBuf pos, vel, force, g_owner, g_code, cell_of, cell_slot, cell_inv, parts, spos;
-    DevBuf cell_count, cell_start, tile_count, tile_start, work;
+    int threads = kTileThreads;      // tile block size in use (cfg.threads, or automatic at set-up)
+    DevBuf dR, sb_rep, eb_rep, cb_rep, gb_rep, tile_rep, cell_rep;
+    DevBuf pos, vel, force, g_owner, g_code, cell_of, cell_slot, cell_inv, parts, spos, spos2;
+    DevBuf cell_count, cell_start, tile_count, tile_start, work, work_k;
     DevBuf sb_cnt0, sb_cnt1, scan_tmp, nb_count, nb_list;
     DevBuf dyn, flags, epart, kpart, tout;
     DevBuf snap_pos, snap_vel, snap_force;
@@ -595,9 +626,9 @@ struct Batch {
 
     ~Batch() {
         drop_graphs();
-        DevBuf* bufs[] = {&dR, &sb_rep, &eb_rep, &cb_rep, &tile_rep, &cell_rep, &pos, &vel, &force, &g_owner,
-                          &g_code, &cell_of, &cell_slot, &cell_inv, &parts, &spos, &cell_count, &cell_start, &tile_count,
-                          &tile_start, &work, &sb_cnt0, &sb_cnt1, &scan_tmp, &nb_count, &nb_list, &dyn, &flags, &epart,
+        DevBuf* bufs[] = {&dR, &sb_rep, &eb_rep, &cb_rep, &gb_rep, &tile_rep, &cell_rep, &pos, &vel, &force, &g_owner,
+                          &g_code, &cell_of, &cell_slot, &cell_inv, &parts, &spos, &spos2, &cell_count, &cell_start,
+                          &tile_count, &tile_start, &work, &work_k, &sb_cnt0, &sb_cnt1, &scan_tmp, &nb_count, &nb_list, &dyn, &flags, &epart,
                           &kpart, &tout, &snap_pos, &snap_vel, &snap_force};
         if (stream) (void)hipStreamSynchronize(stream);
         for (DevBuf* b : bufs) b->release();
@@ -631,7 +662,7 @@ static void layout(Batch& B) {
     B.R.assign(B.nrep, RepDesc{});
     const double cut = B.rc + B.skin;
     const double spacing = cut * B.cfg.spacing;
-    std::vector<int> sb, eb, cb, tr, cr;
+    std::vector<int> sb, eb, cb, gb, tr, cr;
     int a0 = 0, w0 = 0, c0 = 0, t0 = 0;
     for (int r = 0; r < B.nrep; ++r) {
         RepDesc& D = B.R[r];
@@ -658,9 +689,11 @@ static void layout(Batch& B) {
         D.sb0 = (int)sb.size(); D.nsb = div_up(D.cap, kScanTile);
         D.eb0 = (int)eb.size(); D.neb = std::max(1, div_up(D.n, kBlk));
         D.cb0 = (int)cb.size(); D.ncb = div_up(D.cap, kBlk);
+        D.gb0 = (int)gb.size(); D.ngb = div_up(D.cap - D.n, kBlk);
         for (int k = 0; k < D.nsb; ++k) sb.push_back(r);
         for (int k = 0; k < D.neb; ++k) eb.push_back(r);
         for (int k = 0; k < D.ncb; ++k) cb.push_back(r);
+        for (int k = 0; k < D.ngb; ++k) gb.push_back(r);
         for (int k = 0; k < D.ntiles; ++k) tr.push_back(r);
         for (int k = 0; k < D.ncells; ++k) cr.push_back(r);
         for (int d = 0; d < 6; ++d) D.aabb[d] = G.aabb[d];
@@ -687,6 +720,7 @@ static void layout(Batch& B) {
     B.n_sb = (int)sb.size();
     B.n_eb = (int)eb.size();
     B.n_cb = (int)cb.size();
+    B.n_gb = (int)gb.size();
     hipStream_t st = B.stream;
     auto upload_ints = [&](DevBuf& b, const std::vector<int>& v) {
         b.reserve(sizeof(int) * std::max<size_t>(v.size(), 1), false, st);
@@ -697,6 +731,7 @@ static void layout(Batch& B) {
     upload_ints(B.sb_rep, sb);
     upload_ints(B.eb_rep, eb);
     upload_ints(B.cb_rep, cb);
+    upload_ints(B.gb_rep, gb);
     upload_ints(B.tile_rep, tr);
     upload_ints(B.cell_rep, cr);
     const size_t S = (size_t)B.n_slots;
@@ -713,11 +748,13 @@ static void layout(Batch& B) {
     B.cell_inv.reserve(sizeof(int) * S, false, st);
     B.parts.reserve(sizeof(int) * S, false, st);
     B.spos.reserve(sizeof(double4) * S, false, st);
+    B.spos2.reserve(sizeof(double4) * S, false, st);
     B.cell_count.reserve(sizeof(int) * (size_t)B.n_cells, false, st);
     B.cell_start.reserve(sizeof(int) * ((size_t)B.n_cells + 1), false, st);
     B.tile_count.reserve(sizeof(int) * (size_t)B.n_tiles, false, st);
     B.tile_start.reserve(sizeof(int) * ((size_t)B.n_tiles + 1), false, st);
     B.work.reserve(sizeof(int) * ((size_t)B.n_work + 1), false, st);
+    B.work_k.reserve(sizeof(int) * ((size_t)B.n_work + 1), false, st);
     B.sb_cnt0.reserve(sizeof(int) * (size_t)B.n_sb, false, st);
     B.sb_cnt1.reserve(sizeof(int) * (size_t)B.n_sb, false, st);
     B.scan_tmp.reserve(sizeof(int) * (size_t)scan_scratch(std::max(B.n_cells, B.n_tiles)), false, st);
@@ -794,21 +831,28 @@ static void enq_rebuild(Batch& B) {
                        B.tile_count.as<int>(), B.flags.as<int>());
     scan_exclusive(B.tile_count.as<int>(), B.tile_start.as<int>(), B.n_tiles, B.tile_start.as<int>() + B.n_tiles,
                    B.scan_tmp.as<int>(), st);
-    hipLaunchKernelGGL(k_b_tile_fill, dim3(div_up(B.n_tiles, kBlk)), dim3(kBlk), 0, st, R, (const int*)B.tile_rep.as<int>(),
+    hipLaunchKernelGGL(k_b_tile_fill, dim3(B.n_tiles), dim3(64), 0, st, R, (const int*)B.tile_rep.as<int>(),
                        (const int*)B.cell_start.as<int>(), (const int*)B.cell_count.as<int>(),
-                       (const int*)B.tile_start.as<int>(), B.n_tiles, (const int*)B.parts.as<int>(), B.work.as<int>());
+                       (const int*)B.tile_start.as<int>(), (const int*)B.parts.as<int>(), B.work.as<int>(),
+                       B.work_k.as<int>());
     hipLaunchKernelGGL(k_b_check_work, dim3(div_up(B.nrep, kBlk)), dim3(kBlk), 0, st, R, B.nrep,
                        (const int*)B.tile_start.as<int>(), B.flags.as<int>());
     TMD_HIP(hipGetLastError());
 }
 
-static void enq_mirror(Batch& B) {
+static void enq_mirror(Batch& B, DevBuf& m) {
     hipLaunchKernelGGL(k_b_mirror, dim3(div_up(B.n_slots, kBlk)), dim3(kBlk), 0, B.stream, (const double4*)B.pos.as<double4>(),
-                       (const int*)B.parts.as<int>(), (const int*)(B.cell_start.as<int>() + B.n_cells), B.spos.as<double4>(),
+                       (const int*)B.parts.as<int>(), (const int*)(B.cell_start.as<int>() + B.n_cells), m.as<double4>(),
                        B.n_slots);
 }
 
-static void enq_nlist(Batch& B) {
+static void enq_unmirror(Batch& B, const DevBuf& m) {
+    hipLaunchKernelGGL(k_b_unmirror, dim3(B.n_eb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
+                       (const int*)B.eb_rep.as<int>(), (const int*)B.cell_inv.as<int>(), (const double4*)m.as<double4>(),
+                       B.pos.as<double4>());
+}
+
+static void enq_nlist(Batch& B, const DevBuf& m) {
     NbArgs a{};
     a.cap = B.list_cap;
     a.half = 0;
@@ -820,8 +864,8 @@ static void enq_nlist(Batch& B) {
     const double cut = B.rc + B.skin;
     a.cut2 = cut * cut;
     const size_t lds = (size_t)a.wcap * kStageBytes + rsb;
-    hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.tile_rep.as<int>(), (const double4*)B.spos.as<double4>(), (const double4*)B.pos.as<double4>(),
+    hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
+                       (const int*)B.tile_rep.as<int>(), (const double4*)m.as<double4>(), (const double4*)B.pos.as<double4>(),
                        (const int*)B.cell_of.as<int>(), (const int*)B.cell_inv.as<int>(), (const int*)B.cell_start.as<int>(),
                        (const int*)B.parts.as<int>(),
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), a, B.nb_count.as<int>(),
@@ -829,34 +873,28 @@ static void enq_nlist(Batch& B) {
     TMD_HIP(hipGetLastError());
 }
 
-static void enq_step_refresh(Batch& B) {
-    hipLaunchKernelGGL(k_b_step_refresh, dim3(B.n_cb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.cb_rep.as<int>(), (const int*)B.dyn.as<int>(), B.pos.as<double4>(),
+static void enq_ghost_step(Batch& B, DevBuf& m) {
+    hipLaunchKernelGGL(k_b_ghost_step, dim3(B.n_gb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
+                       (const int*)B.gb_rep.as<int>(), (const int*)B.dyn.as<int>(), B.pos.as<double4>(),
                        (const int*)B.g_owner.as<int>(), (const int*)B.g_code.as<int>(), (const int*)B.cell_inv.as<int>(),
-                       B.spos.as<double4>());
+                       m.as<double4>());
 }
 
 // kick: final kick v += F dt/2 (hdt = 0 leaves v unchanged: v + f * 0 == v);
-// advance: also the next step's kick and drift
-static void enq_force(Batch& B, bool kick, bool advance) {
+// advance: also the next step's kick and drift, into the mirror `next`
+static void enq_force(Batch& B, bool kick, bool advance, const DevBuf& cur, DevBuf& next) {
     const LJArgs a = batch_lj_args(B);
     const size_t lds = (size_t)B.wcap * kStageBytes;
     const bool unit = B.eps == 1.0 && B.sigma == 1.0;
     const double hdt = kick ? 0.5 * B.dt : 0.0;
-#define TMD_B_LJ(P, SH, U)                                                                                          \
-    hipLaunchKernelGGL((k_b_lj_full<P, SH, U, 1>), dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,              \
-                       (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
-                       (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), \
+#define TMD_B_LJ_ADV(P, SH, U, ADV)                                                                                 \
+    hipLaunchKernelGGL((k_b_lj_full<P, SH, U, ADV>), dim3(B.n_tiles), dim3(B.threads), lds, B.stream,                \
+                       (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)cur.as<double4>(), \
+                       (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), (const int*)B.work_k.as<int>(), \
                        (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), (const nb_t*)B.nb_list.as<nb_t>(), \
-                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, B.pos.as<double4>())
-#define TMD_B_LJ0(P, SH, U)                                                                                         \
-    hipLaunchKernelGGL((k_b_lj_full<P, SH, U, 0>), dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,              \
-                       (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
-                       (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), \
-                       (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), (const nb_t*)B.nb_list.as<nb_t>(), \
-                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, B.pos.as<double4>())
+                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, next.as<double4>())
 #define TMD_B_LJ_A(P, SH, U) \
-    do { if (advance) TMD_B_LJ(P, SH, U); else TMD_B_LJ0(P, SH, U); } while (0)
+    do { if (advance) TMD_B_LJ_ADV(P, SH, U, 1); else TMD_B_LJ_ADV(P, SH, U, 0); } while (0)
 #define TMD_B_LJ_U(P, SH) \
     do { if (unit) TMD_B_LJ_A(P, SH, 1); else TMD_B_LJ_A(P, SH, 0); } while (0)
 #define TMD_B_LJ_S(P) \
@@ -867,8 +905,7 @@ static void enq_force(Batch& B, bool kick, bool advance) {
 #undef TMD_B_LJ_S
 #undef TMD_B_LJ_U
 #undef TMD_B_LJ_A
-#undef TMD_B_LJ0
-#undef TMD_B_LJ
+#undef TMD_B_LJ_ADV
     TMD_HIP(hipGetLastError());
 }
 
@@ -881,19 +918,29 @@ static void enq_int1(Batch& B) {
 
 // the main.cpp step loop over `steps` steps (rebuild when (j + 1) % reneigh
 // == 0); each force pass also performs the final kick of its step and the
-// first kick + drift of the next, so only the first step integrates alone
+// first kick + drift of the next, so only the first step integrates alone.
+// Positions live in the two mirrors during the run (the force pass reads one
+// and writes the other; the enqueue alternates them); pos is restored from
+// the mirror before each rebuild and at the end.  The first integration
+// writes pos and the mirror, so a run depends only on pos, vel and force.
 static void enq_steps(Batch& B, int steps) {
+    DevBuf* cur = &B.spos;
+    DevBuf* nxt = &B.spos2;
     if (steps > 0) enq_int1(B);
     for (int j = 0; j < steps; ++j) {
         if ((j + 1) % B.reneigh == 0) {
+            enq_unmirror(B, *cur);
             enq_rebuild(B);
-            enq_mirror(B);
-            enq_nlist(B);
+            enq_mirror(B, *cur);
+            enq_nlist(B, *cur);
         } else {
-            enq_step_refresh(B);
+            enq_ghost_step(B, *cur);
         }
-        enq_force(B, true, j + 1 < steps);
+        const bool adv = j + 1 < steps;
+        enq_force(B, true, adv, *cur, *nxt);
+        if (adv) std::swap(cur, nxt);
     }
+    if (steps > 0) enq_unmirror(B, *cur);
     TMD_HIP(hipGetLastError());
 }
 
@@ -955,20 +1002,21 @@ static bool setup_lists(Batch& B) {
             return false;
         }
         const int maxwin = fl[FL_MAXWIN];
+        B.threads = tile_block_threads(B.cfg.threads, fl[FL_MAXLOC]);
         // capacities only grow, so a restart keeps a captured run valid
         B.wcap = std::max(B.wcap, std::min(maxwin + maxwin / 4 + 64, kLdsBudget / kStageBytes));
-        enq_mirror(B);
+        enq_mirror(B, B.spos);
         for (int k = 0; k < 4 && !has_error(); ++k) {
             reserve_lists(B);
             fill(B, B.flags, FL_N, 0);
-            enq_nlist(B);
+            enq_nlist(B, B.spos);
             read_flags(B, fl);
             if (fl[FL_LIST] == 0) break;
             B.list_cap = std::max(B.list_cap, ((int)std::ceil(fl[FL_LIST] * B.cfg.list_headroom) + 7) & ~7);
         }
         if (has_error()) return false;
         if (fl[FL_LIST] > 0) continue;
-        enq_force(B, false, false);
+        enq_force(B, false, false, B.spos, B.spos2);
         TMD_HIP(hipStreamSynchronize(B.stream));
         B.ready = !has_error();
         return B.ready;
@@ -995,9 +1043,10 @@ static bool setup_from_initial(Batch& B, bool fresh = true) {
 // Everything a captured run bakes in: buffer addresses and launch scalars.
 static std::vector<unsigned long long> graph_signature(const Batch& B) {
     std::vector<unsigned long long> s;
-    const DevBuf* bufs[] = {&B.dR, &B.sb_rep, &B.eb_rep, &B.cb_rep, &B.tile_rep, &B.cell_rep, &B.pos, &B.vel, &B.force,
-                            &B.g_owner, &B.g_code, &B.cell_of, &B.cell_slot, &B.cell_inv, &B.parts, &B.spos,
-                            &B.cell_count, &B.cell_start, &B.tile_count, &B.tile_start, &B.work, &B.sb_cnt0, &B.sb_cnt1,
+    const DevBuf* bufs[] = {&B.dR, &B.sb_rep, &B.eb_rep, &B.cb_rep, &B.gb_rep, &B.tile_rep, &B.cell_rep, &B.pos, &B.vel,
+                            &B.force, &B.g_owner, &B.g_code, &B.cell_of, &B.cell_slot, &B.cell_inv, &B.parts, &B.spos,
+                            &B.spos2, &B.cell_count, &B.cell_start, &B.tile_count, &B.tile_start, &B.work, &B.work_k,
+                            &B.sb_cnt0, &B.sb_cnt1,
                             &B.scan_tmp, &B.nb_count, &B.nb_list, &B.dyn, &B.flags};
     for (const DevBuf* b : bufs) s.push_back((unsigned long long)(uintptr_t)b->p);
     const double dv[] = {B.rc, B.skin, B.eps, B.sigma, B.dt};
@@ -1006,7 +1055,7 @@ static std::vector<unsigned long long> graph_signature(const Batch& B) {
         std::memcpy(&u, &d, sizeof u);
         s.push_back(u);
     }
-    const int iv[] = {B.reneigh, B.precision, B.shift, B.list_cap, B.wcap, B.cfg.threads, B.n_sb, B.n_eb, B.n_cb,
+    const int iv[] = {B.reneigh, B.precision, B.shift, B.list_cap, B.wcap, B.threads, B.n_sb, B.n_eb, B.n_cb, B.n_gb,
                       B.n_tiles, B.n_cells, B.n_slots, B.nrep};
     for (int v : iv) s.push_back((unsigned long long)(unsigned)v);
     return s;
@@ -1115,9 +1164,9 @@ static bool run_steps(Batch& B, int steps, double* gpu_ms) {
 static void batch_thermo(Batch& B, double* out) {
     const LJArgs a = batch_lj_args(B);
     const size_t lds = (size_t)B.wcap * kStageBytes;
-    enq_mirror(B);
+    enq_mirror(B, B.spos);
 #define TMD_B_EN(SH)                                                                                                 \
-    hipLaunchKernelGGL(k_b_energy<SH>, dim3(B.n_tiles), dim3(B.cfg.threads), lds, B.stream,                          \
+    hipLaunchKernelGGL(k_b_energy<SH>, dim3(B.n_tiles), dim3(B.threads), lds, B.stream,                          \
                        (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
                        (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.parts.as<int>(), \
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), \
@@ -1239,7 +1288,7 @@ int tmd_batch_setup(int h, double cutoff, double skin, double epsilon, double si
 int tmd_batch_configure(int h, int tile, int threads, double spacing_factor, double list_headroom, double ghost_headroom) {
     Batch* B = find_batch(h);
     if (!B || has_error()) return 0;
-    if (threads < 64 || threads > kTileThreads || threads % 64 != 0 || tile < 0 || tile > kMaxTile ||
+    if ((threads != 0 && (threads < 64 || threads > kMaxTileThreads || threads % 64 != 0)) || tile < 0 || tile > kMaxTile ||
         !(spacing_factor >= 1.0) || !(list_headroom >= 1.0) || !(ghost_headroom > 0.0)) {
         set_error(__FILE__, __LINE__, "batch: invalid configuration");
         return 0;
@@ -1271,7 +1320,7 @@ int tmd_batch_autotune(int h, int trial_steps, double* out) {
     if (trial_steps <= 0) trial_steps = 2 * B->reneigh;
     const BatchCfg keep = B->cfg;
     const int tiles[] = {0, 2, 3, 4};
-    const int threads[] = {256, 512};
+    const int threads[] = {0, 256, 512};  // 0: one lane per local of the fullest tile
     const double spacing[] = {1.0, 1.1};
     const double headroom[] = {1.1, 1.3};
     double best = 1e300;
@@ -1301,7 +1350,7 @@ int tmd_batch_autotune(int h, int trial_steps, double* out) {
     if (!setup_from_initial(*B)) return 0;
     if (out) {
         out[0] = best_cfg.tile;
-        out[1] = best_cfg.threads;
+        out[1] = B->threads;  // the block size in use (automatic when the configuration says 0)
         out[2] = best_cfg.spacing;
         out[3] = best_cfg.list_headroom;
         out[4] = best / trial_steps;

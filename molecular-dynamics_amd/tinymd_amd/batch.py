@@ -86,7 +86,7 @@ class Batch:
     def nrep(self):
         return len(self.counts)
 
-    def configure(self, tile=0, threads=512, spacing=1.0, list_headroom=1.2, ghost_headroom=1.5):
+    def configure(self, tile=0, threads=0, spacing=1.0, list_headroom=1.2, ghost_headroom=1.5):
         self.L.tmd_batch_configure(self.h, tile, threads, spacing, list_headroom, ghost_headroom)
         check()
 

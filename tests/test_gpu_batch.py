@@ -75,7 +75,7 @@ def test_batch_thermo_vs_single(tm, boxes):
     b.close()
 
 
-@pytest.mark.parametrize("tile,threads", [(2, 256), (4, 512), (1, 64)])
+@pytest.mark.parametrize("tile,threads", [(2, 256), (4, 512), (1, 64), (3, 0), (3, 768)])
 def test_batch_launch_config_invariance(tm, boxes, tile, threads):
     announce(f"batch config tile={tile} threads={threads}")
     b = tm.Batch(boxes, precision="fp64")
@@ -120,7 +120,8 @@ def test_batch_autotune_keeps_results(tm, boxes):
     announce("batch autotune")
     b = tm.Batch(boxes, precision="fp32mixed")
     cfg = b.autotune(trial_steps=20)
-    assert cfg["threads"] in (256, 512) and cfg["spacing"] in (1.0, 1.1) and cfg["ms_per_step"] > 0
+    assert cfg["threads"] % 64 == 0 and 64 <= cfg["threads"] <= 1024
+    assert cfg["spacing"] in (1.0, 1.1) and cfg["ms_per_step"] > 0
     b.run(45)
     assert_bitwise(tm, b, boxes, 45, "fp32mixed", spacing=cfg["spacing"])
     b.close()
