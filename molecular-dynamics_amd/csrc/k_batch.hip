@@ -533,18 +533,14 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_energy(const RepDesc* __r
     const RepDesc& D = R[tile_rep[tile]];
     __shared__ TileWin W;
     tile_window(tile - D.t0, start + D.c0, D.g, W);
-    double* sx = dyn;
-    double* sy = dyn + a.wcap;
-    double* sz = dyn + 2 * a.wcap;
-    int* sj = reinterpret_cast<int*>(dyn + 3 * a.wcap);
-    const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
+    const WinStage sp = win_stage(dyn, a.wcap);
+    const bool staged = stage_window(W, a.wcap, spos, parts, sp);
     double pe = 0.0, vir = 0.0, np = 0.0;
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
-        if (staged) lj_energy_atom<SHIFT, true>(pos[work[w]], w, nbc[w], nbl, a, 0, D.a0 + D.n, sx, sy, sz, sj, spos, parts, W, pe, vir,
-                              np);
-        else lj_energy_atom<SHIFT, false>(pos[work[w]], w, nbc[w], nbl, a, 0, D.a0 + D.n, sx, sy, sz, sj, spos, parts, W, pe, vir,
-                              np);
+        const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+        if (staged) lj_energy_atom<SHIFT, true>(pos[work[w]], nbc[w], lp, sp, spos, parts, W, a, 0, D.a0 + D.n, pe, vir, np);
+        else lj_energy_atom<SHIFT, false>(pos[work[w]], nbc[w], lp, sp, spos, parts, W, a, 0, D.a0 + D.n, pe, vir, np);
     }
     const double bpe = block_sum_double_any(pe);
     const double bvir = block_sum_double_any(vir);

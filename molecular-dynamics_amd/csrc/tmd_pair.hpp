@@ -174,42 +174,49 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
 // Energy and virial terms of one atom's list entries (FP64): U = 4 eps
 // (sr12 - sr6) [+ shift terms], W = f r^2.  Full lists weight every entry
 // 1/2; half lists weight local-local entries 1 and local-ghost entries 1/2
-// (an index j >= nloc_end is a ghost).
+// (an index j >= nloc_end is a ghost).  Entries are read 8 at a time (one
+// 16-byte load per group, as in the force pass) and summed in list order.
 template <int SHIFT, bool STAGED>
-__device__ __forceinline__ void lj_energy_atom(const double4 pi, int w, int cnt, const nb_t* __restrict__ nbl, const LJArgs& a,
-                                               int half, int nloc_end, const double* sx, const double* sy,
-                                               const double* sz, const int* sj, const double4* __restrict__ spos,
-                                               const int* __restrict__ parts, const TileWin& W, double& pe, double& vir,
-                                               double& np) {
-    for (int k = 0; k < cnt; ++k) {
-        const int t = nbl[nb16_slot(w, k, a.cap)];
-        int j;
-        double xj, yj, zj;
-        if (STAGED) {
-            j = sj[t];
-            xj = sx[t]; yj = sy[t]; zj = sz[t];
-        } else {
-            const int kk = win_index(t, W);
-            j = parts[kk];
-            const double4 p = spos[kk];
-            xj = p.x; yj = p.y; zj = p.z;
-        }
-        const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        if (rsq < a.rc2) {
-            const double sr2 = 1.0 / rsq;
-            const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
-            double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
-            double u = 4.0 * a.eps * sr6 * (sr6 - 1.0);
-            if (SHIFT) {
-                const double ds = rsq - a.rc2;
-                f = f - a.fc - ds * a.dfc;
-                u = u - a.uc + ds * (0.5 * a.fc) + ds * ds * (0.25 * a.dfc);
+__device__ __forceinline__ void lj_energy_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const WinStage& sp,
+                                               const double4* __restrict__ spos, const int* __restrict__ parts,
+                                               const TileWin& W, const LJArgs& a, int half, int nloc_end, double& pe,
+                                               double& vir, double& np) {
+    const int ng = (cnt + 7) >> 3;
+    for (int gi = 0; gi < ng; ++gi) {
+        const int4 e = lp[(size_t)gi * 64];
+        const int kend = cnt - gi * 8;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (u >= kend) break;
+            const int t = nb_entry(e, u);
+            int j;
+            double xj, yj, zj;
+            if (STAGED) {
+                j = sp.j[t];
+                xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
+            } else {
+                const int kk = win_index(t, W);
+                j = parts[kk];
+                const double4 p = spos[kk];
+                xj = p.x; yj = p.y; zj = p.z;
             }
-            const double wt = half ? (j < nloc_end ? 1.0 : 0.5) : 0.5;
-            pe += wt * u;
-            vir += wt * (f * rsq);
-            np += 1.0;
+            const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq < a.rc2) {
+                const double sr2 = 1.0 / rsq;
+                const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
+                double f = 48.0 * sr6 * (sr6 - 0.5) * sr2 * a.eps;
+                double uu = 4.0 * a.eps * sr6 * (sr6 - 1.0);
+                if (SHIFT) {
+                    const double ds = rsq - a.rc2;
+                    f = f - a.fc - ds * a.dfc;
+                    uu = uu - a.uc + ds * (0.5 * a.fc) + ds * ds * (0.25 * a.dfc);
+                }
+                const double wt = half ? (j < nloc_end ? 1.0 : 0.5) : 0.5;
+                pe += wt * uu;
+                vir += wt * (f * rsq);
+                np += 1.0;
+            }
         }
     }
 }

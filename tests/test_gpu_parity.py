@@ -275,3 +275,24 @@ def test_trajectory_bitwise_16cube_staged_windows(tm):
     co, lo = orc.neighbor_lists()
     assert np.array_equal(cg, co)
     assert all(np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]) for i in range(0, len(cg), 97))
+
+
+@pytest.mark.parametrize("tile,threads", [((3, 3, 3), 64), ((2, 2, 2), 1024), ((4, 4, 2), 0), ((8, 8, 8), 256)])
+def test_trajectory_bitwise_tile_shapes(tm, tile, threads):
+    """Blocks smaller than a tile's local count (several passes per block),
+    larger than it, the automatic size, and windows beyond the LDS budget
+    (read from the mirror): the FP64 trajectory across a rebuild is the
+    oracle's bit for bit."""
+    announce(f"trajectory 16^3, 25 steps, tile {tile}, threads {threads}")
+    aabb, m, x, v = perturbed_fcc(16, amp=0.05, seed=22)
+    sim, orc = make_pair(aabb, m, x, v, threads=8)
+    sim.set_tile(*tile, threads=threads)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    sim.run_lj(25, reneigh=20, dt=0.005)
+    orc.run_lj(25, reneigh=20, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    sim.close()
