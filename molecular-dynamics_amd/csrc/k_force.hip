@@ -52,7 +52,8 @@ namespace tmd {
 template <int PREC, int SHIFT, int UNIT, int KICK>
 __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
                                                          double4* __restrict__ vel, double hdt, double dt,
-                                                         const int* __restrict__ wk, double4* __restrict__ spos_next) {
+                                                         const int* __restrict__ wk, double4* __restrict__ spos_next,
+                                                         int vel_by_k) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -73,7 +74,8 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
         if (KICK != 2) force[i] = f;
         if (KICK) {
             // velocity-Verlet final kick (k_final_integration's arithmetic)
-            double4 v = vel[i];
+            const int iv = vel_by_k ? kw : i;  // velocities in atom or in mirror order
+            double4 v = vel[iv];
             v.x = v.x + f.x * hdt;
             v.y = v.y + f.y * hdt;
             v.z = v.z + f.z * hdt;
@@ -88,7 +90,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
                 x.z = x.z + v.z * dt;
                 spos_next[kw] = x;
             }
-            vel[i] = v;
+            vel[iv] = v;
         }
     }
 }
@@ -466,20 +468,22 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         const size_t lds = lds_bytes(s, false);
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
         const double hdt = 0.5 * kick_dt;
-        double4* V = s.vel.as<double4>();
+        // inside the fused step loop the velocities live in mirror order
+        double4* V = s.vel_mirrored ? s.svel.as<double4>() : s.vel.as<double4>();
+        const int VK = s.vel_mirrored;
         const int* WK = s.work_k.as<int>();
         double4* SN = s.spos2.as<double4>();
 #define TMD_LJ_FULL(P, SH, U)                                                                                          \
     do {                                                                                                               \
         if (kick_dt > 0.0 && advance)                                                                                  \
             hipLaunchKernelGGL((k_lj_full<P, SH, U, 2>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, WK, SN);                                                                       \
+                               kick_dt, WK, SN, VK);                                                                    \
         else if (kick_dt > 0.0)                                                                                        \
             hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, WK, SN);                                                                       \
+                               kick_dt, WK, SN, VK);                                                                    \
         else                                                                                                           \
             hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, WK, SN);                                                                       \
+                               kick_dt, WK, SN, VK);                                                                    \
     } while (0)
         if (s.precision == PREC_FP64) {
             if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }

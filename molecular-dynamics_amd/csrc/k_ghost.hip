@@ -381,6 +381,32 @@ void sync_pos_from_mirror(Sim& s) {
     s.pos_stale = 0;
 }
 
+// velocities of the locals between atom order (vel) and mirror order (svel)
+__global__ void k_vel_mirror(const int* __restrict__ work, const int* __restrict__ wk, int n, double4* __restrict__ vel,
+                             double4* __restrict__ svel, int into_mirror) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    if (into_mirror) svel[wk[w]] = vel[work[w]];
+    else vel[work[w]] = svel[wk[w]];
+}
+
+// The fused step loop keeps the locals' velocities in mirror order (the
+// force pass then reads and writes them with the same coalesced index as the
+// positions); they return to atom order before each rebuild and at loop exit.
+void mirror_velocities(Sim& s, bool into_mirror) {
+    if (has_error() || s.n == 0) return;
+    if (into_mirror == (s.vel_mirrored != 0)) return;
+    if (into_mirror) {
+        s.svel.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+        if (has_error()) return;
+    }
+    const int nt = 256;
+    hipLaunchKernelGGL(k_vel_mirror, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, (const int*)s.work.as<int>(),
+                       (const int*)s.work_k.as<int>(), s.n, s.vel.as<double4>(), s.svel.as<double4>(), into_mirror ? 1 : 0);
+    TMD_HIP(hipGetLastError());
+    s.vel_mirrored = into_mirror ? 1 : 0;
+}
+
 void step_refresh(Sim& s) {
     if (has_error() || s.n + s.ng == 0) return;
     if (!s.inv_valid || s.ghost_mode != GHOST_FRESH) {

@@ -160,6 +160,8 @@ static void upload(Sim& s) {
         v[i] = make_double4(s.h_vel[3 * i], s.h_vel[3 * i + 1], s.h_vel[3 * i + 2], 0.0);
     }
     mirror_invalidate(s);
+    s.pos_stale = 0;  // pos and vel are uploaded whole
+    s.vel_mirrored = 0;
     TMD_HIP(hipMemcpyAsync(s.pos.p, p.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.vel.p, v.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.mass.p, s.h_mass.data(), sizeof(double) * nall, hipMemcpyHostToDevice, s.stream));
@@ -303,6 +305,8 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     s.windows_valid = 0;
     s.inv_valid = 0;
     mirror_invalidate(s);
+    s.pos_stale = 0;
+    s.vel_mirrored = 0;
     s.h_mass.assign(masses, masses + nparticles);
     s.h_pos.resize(3 * (size_t)nparticles);
     s.h_vel.resize(3 * (size_t)nparticles);
@@ -434,7 +438,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.tile_start, &s.spos, &s.spos2, &s.face,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel,
                       &s.nb_count, &s.nb_list,
                       &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
@@ -446,6 +450,8 @@ void md_deallocate_grid() {
     s.windows_valid = 0;
     s.inv_valid = 0;
     mirror_invalidate(s);
+    s.pos_stale = 0;
+    s.vel_mirrored = 0;
     s.nbinned = 0;
     s.dev_valid = false;
 }
@@ -771,10 +777,13 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         // the first kick + drift of the next one, so a step is a ghost/mirror
         // refresh (or a rebuild) and one force pass
         if (steps > 0) initial_integration(s, dt);
+        if (!hf) mirror_velocities(s, true);
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {
-                sync_pos_from_mirror(s);  // the rebuild reads pos
+                sync_pos_from_mirror(s);  // the rebuild reads pos and vel
+                mirror_velocities(s, false);
                 rebuild(s, hf, cutoff + skin);
+                if (!hf) mirror_velocities(s, true);
             } else {
                 hipEvent_t ev = nullptr;
                 s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
@@ -785,6 +794,7 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
             ++done;
         }
         sync_pos_from_mirror(s);
+        mirror_velocities(s, false);
     } else {
         for (int j = 0; j < steps && !has_error(); ++j) {
             initial_integration(s, dt);

@@ -103,6 +103,7 @@ struct Sim {
     int ghost_stale = 0;    // the last force pass advanced the locals in the mirror; the ghosts (pos and
                             // mirror) still hold the previous step
     int pos_stale = 0;      // local positions in pos lag the mirror (fused step loop; sync_pos_from_mirror)
+    int vel_mirrored = 0;   // local velocities live in svel, in mirror order (fused step loop)
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
@@ -129,6 +130,7 @@ struct Sim {
     DevBuf spos2;                      // second mirror: the force pass that advances positions writes the
                                        // new local positions here, then the two are swapped
     DevBuf work_k;                     // mirror (CSR) index of each work slot's atom
+    DevBuf svel;                       // local velocities in mirror order during the fused step loop
     DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
     DevBuf nb_count, nb_list;
     DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
@@ -175,6 +177,7 @@ void refresh_ghosts(Sim& s);
 // one pass (after positions were advanced inside the force pass)
 void step_refresh(Sim& s);
 void sync_pos_from_mirror(Sim& s);
+void mirror_velocities(Sim& s, bool into_mirror);
 
 // geometry (core/grid.impala:94-98): world AABB, rank AABB, cell spacing
 void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell_spacing);
