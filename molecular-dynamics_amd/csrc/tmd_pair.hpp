@@ -235,8 +235,8 @@ struct NbArgs {
 // accepted); i itself is removed from the mask by its window offset, so full
 // lists read no atom index (half lists read the staged index for the j < i /
 // ghost filter).  The set bits are then appended in ascending order, which is
-// the reference's order, packed 8 to a 16-byte group in registers, and each
-// full group is written with one store (one list column per lane).  Only
+// the reference's order, funnel-shifted 8 to a 16-byte group in registers,
+// and each full group is written with one store (one list column per lane).  Only
 // accepted candidates pay for the packing.  The last group's free slots hold
 // the window's sentinel offset.  Returns the count; groups beyond the
 // capacity are counted, not stored.  STAGED is a template parameter so that
@@ -254,7 +254,9 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
     const int t_i = W.seg_off[s_own] + (k_i - W.seg_beg[s_own]);
     int4* const col = reinterpret_cast<int4*>(nb_list) + (size_t)(w >> 6) * (size_t)a.cap * 8 + (w & 63);
     const int ngroups = a.cap >> 3;
-    unsigned long long lo = 0ull, hi = 0ull;
+    // the group being filled: entries are shifted in at the top (funnel
+    // shifts), so after 8 appends the oldest sits in the low half of q0
+    unsigned q0 = 0u, q1 = 0u, q2 = 0u, q3 = 0u;
     int cnt = 0;
     for (int r = 0; r < 9; ++r) {
         const int s = s_own + (r / 3 - 1) * wy + (r % 3 - 1);
@@ -292,32 +294,30 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
             const unsigned d = (unsigned)(t_i - tc);
             m &= ~(d < 32u ? 1u << d : 0u);  // i is not its own neighbour
             while (m != 0u) {
-                const int t = tc + __builtin_ctz(m);
+                const unsigned t = (unsigned)(tc + __builtin_ctz(m));
                 m &= m - 1u;
-                const int p = cnt & 7;
-                const unsigned long long v = (unsigned long long)(unsigned)t << (16 * (p & 3));
-                lo |= p < 4 ? v : 0ull;
-                hi |= p >= 4 ? v : 0ull;
+                q0 = __builtin_amdgcn_alignbit(q1, q0, 16);
+                q1 = __builtin_amdgcn_alignbit(q2, q1, 16);
+                q2 = __builtin_amdgcn_alignbit(q3, q2, 16);
+                q3 = __builtin_amdgcn_alignbit(t, q3, 16);
                 ++cnt;
                 if ((cnt & 7) == 0) {
                     const int grp = (cnt >> 3) - 1;
-                    if (grp < ngroups)
-                        col[(size_t)grp * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
-                                                          (int)(unsigned)(hi >> 32));
-                    lo = hi = 0ull;
+                    if (grp < ngroups) col[(size_t)grp * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
                 }
             }
         }
     }
     if ((cnt & 7) != 0 && (cnt >> 3) < ngroups) {
         // the last group's free slots hold the window's sentinel offset
-        const unsigned long long pad = (unsigned long long)(unsigned)win_size(W);
+        const unsigned pad = (unsigned)win_size(W);
         for (int p = cnt & 7; p < 8; ++p) {
-            if (p < 4) lo |= pad << (16 * p);
-            else hi |= pad << (16 * (p - 4));
+            q0 = __builtin_amdgcn_alignbit(q1, q0, 16);
+            q1 = __builtin_amdgcn_alignbit(q2, q1, 16);
+            q2 = __builtin_amdgcn_alignbit(q3, q2, 16);
+            q3 = __builtin_amdgcn_alignbit(pad, q3, 16);
         }
-        col[(size_t)(cnt >> 3) * 64] = make_int4((int)(unsigned)lo, (int)(unsigned)(lo >> 32), (int)(unsigned)hi,
-                                                 (int)(unsigned)(hi >> 32));
+        col[(size_t)(cnt >> 3) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
     }
     return cnt;
 }
