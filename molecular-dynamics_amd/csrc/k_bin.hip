@@ -45,6 +45,29 @@ __global__ void k_bin_scatter(const int* __restrict__ cell_of, const int* __rest
     if (c >= 0) parts[start[c] + slot[i]] = i;
 }
 
+// One thread per cell (sparse grids: a few atoms per cell): insertion sort
+// of the cell segment by atom index, then its locals and the CSR index of
+// every atom; the same result as k_bin_sort.
+__global__ void k_bin_sort_small(const int* __restrict__ start, int ncells, int n, int* __restrict__ parts,
+                                 int* __restrict__ local_count, int* __restrict__ inv) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int b = start[c], e = start[c + 1];
+    for (int k = b + 1; k < e; ++k) {
+        const int v = parts[k];
+        int q = k - 1;
+        while (q >= b && parts[q] > v) {
+            parts[q + 1] = parts[q];
+            --q;
+        }
+        parts[q + 1] = v;
+    }
+    int nl = 0;
+    while (b + nl < e && parts[b + nl] < n) ++nl;
+    local_count[c] = nl;
+    for (int k = b; k < e; ++k) inv[parts[k]] = k;
+}
+
 // One wavefront per cell: sort the cell segment by atom index, then the
 // number of locals in it (locals have the lowest indices, so they lead every
 // sorted segment).
@@ -57,9 +80,10 @@ __global__ void __launch_bounds__(256) k_bin_sort(const int* __restrict__ start,
 }
 
 // Tile work order (see tmd_window.hpp): one thread per tile counts the
-// tile's locals and the size of its window.
-__global__ void k_tile_stats(const int* __restrict__ start, const int* __restrict__ local_count, TileGeom g, int ntiles,
-                             int* __restrict__ tile_count, int* __restrict__ max_window, int* __restrict__ max_locals) {
+// tile's locals and the size of its window (small tiles).
+__global__ void k_tile_stats_small(const int* __restrict__ start, const int* __restrict__ local_count, TileGeom g,
+                                   int ntiles, int* __restrict__ tile_count, int* __restrict__ max_window,
+                                   int* __restrict__ max_locals) {
     const int tile = blockIdx.x * blockDim.x + threadIdx.x;
     if (tile >= ntiles) return;
     const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
@@ -84,6 +108,45 @@ __global__ void k_tile_stats(const int* __restrict__ start, const int* __restric
     }
 }
 
+// The same with one wavefront per tile (large tiles: lanes stride over the
+// cells and the window rows; fixed-order wavefront sums).
+__global__ void __launch_bounds__(kWave) k_tile_stats(const int* __restrict__ start, const int* __restrict__ local_count,
+                                                      TileGeom g, int ntiles, int* __restrict__ tile_count,
+                                                      int* __restrict__ max_window, int* __restrict__ max_locals) {
+    const int tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const int lane = threadIdx.x;
+    const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
+    const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+    const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
+    const int ex = cx1 - cx0, ey = cy1 - cy0, ez = cz1 - cz0;
+    int cnt = 0;
+    for (int q = lane; q < ex * ey * ez; q += kWave)
+        cnt += local_count[((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex)];
+    int S = 0;
+    const int xlo = cx0 - 1, xhi = min(cx0 + g.tx, g.nx - 1);
+    const int wy = g.ty + 2;
+    for (int q = lane; q < wy * (g.tz + 2); q += kWave) {
+        const int y = cy0 - 1 + q % wy, z = cz0 - 1 + q / wy;
+        if (y < g.ny && z < g.nz) {
+            const int row = (z * g.ny + y) * g.nx;
+            S += start[row + xhi + 1] - start[row + xlo];
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        cnt += __shfl_down(cnt, d, kWave);
+        S += __shfl_down(S, d, kWave);
+    }
+    if (lane == 0) {
+        tile_count[tile] = cnt;
+        // read first: only the few tiles that raise a maximum issue the atomic
+        // (thousands of same-address atomics from separate wavefronts serialise)
+        if (cnt > 0 && S > *(volatile int*)max_window) atomicMax(max_window, S);
+        if (cnt > 0 && cnt > *(volatile int*)max_locals) atomicMax(max_locals, cnt);
+    }
+}
+
 // Work order: one block per tile writes the tile's locals in (z, y, x)
 // cell order, one thread per cell (lanes of a wavefront are then atoms of
 // adjacent cells, whose neighbour windows overlap -- measured faster than
@@ -101,14 +164,28 @@ __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__
     const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
     const int ex = cx1 - cx0, ey = cy1 - cy0, ez = cz1 - cz0;
     const int nc = ex * ey * ez;
-    if (threadIdx.x == 0) {
-        int o = 0;
-        for (int q = 0; q < nc; ++q) {
-            const int c = ((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex);
-            coff[q] = o;
-            o += local_count[c];
+    // cell offsets inside the tile: one thread for small tiles, else a
+    // block-wide exclusive scan of the cells' local counts, 256 at a time
+    if (nc <= 64) {
+        if (threadIdx.x == 0) {
+            int o = 0;
+            for (int q = 0; q < nc; ++q) {
+                coff[q] = o;
+                o += local_count[((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex)];
+            }
+            coff[nc] = o;
         }
-        coff[nc] = o;
+    } else {
+        int carry = 0;
+        for (int q0 = 0; q0 < nc; q0 += kScanThreads) {
+            const int q = q0 + (int)threadIdx.x;
+            const int v = q < nc ? local_count[((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex)] : 0;
+            int tot;
+            const int ex_v = block_exclusive_scan(v, &tot);
+            if (q < nc) coff[q] = carry + ex_v;
+            carry += tot;
+        }
+        if (threadIdx.x == 0) coff[nc] = carry;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < nc; q += blockDim.x) {
@@ -172,8 +249,14 @@ void bin_particles(Sim& s) {
     if (nall > 0)
         hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(nall, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_of.as<int>(),
                            (const int*)s.cell_slot.as<int>(), (const int*)s.cell_start.as<int>(), nall, s.cell_parts.as<int>());
-    hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt / kWave)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(), G.ncells,
-                       s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(), s.cell_inv.as<int>());
+    if (nall < 4 * G.ncells)  // sparse grid: a thread per cell beats a wavefront per cell
+        hipLaunchKernelGGL(k_bin_sort_small, dim3(div_up(G.ncells, nt)), dim3(nt), 0, s.stream,
+                           (const int*)s.cell_start.as<int>(), G.ncells, s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(),
+                           s.cell_inv.as<int>());
+    else
+        hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt / kWave)), dim3(nt), 0, s.stream,
+                           (const int*)s.cell_start.as<int>(), G.ncells, s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(),
+                           s.cell_inv.as<int>());
     s.inv_valid = 1;
     // tile work order
     choose_tiles(s);
@@ -184,8 +267,12 @@ void bin_particles(Sim& s) {
     if (has_error()) return;
     int* counters = s.counters.as<int>();
     TMD_HIP(hipMemsetAsync(counters + 6, 0, 2 * sizeof(int), s.stream));
-    hipLaunchKernelGGL(k_tile_stats, dim3(div_up(ntiles, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(),
-                       (const int*)s.cell_count.as<int>(), tg, ntiles, s.work_start.as<int>(), counters + 6, counters + 7);
+    if (tg.tx * tg.ty * tg.tz <= 64)
+        hipLaunchKernelGGL(k_tile_stats_small, dim3(div_up(ntiles, nt)), dim3(nt), 0, s.stream, (const int*)s.cell_start.as<int>(),
+                           (const int*)s.cell_count.as<int>(), tg, ntiles, s.work_start.as<int>(), counters + 6, counters + 7);
+    else
+        hipLaunchKernelGGL(k_tile_stats, dim3(ntiles), dim3(kWave), 0, s.stream, (const int*)s.cell_start.as<int>(),
+                           (const int*)s.cell_count.as<int>(), tg, ntiles, s.work_start.as<int>(), counters + 6, counters + 7);
     exclusive_scan_int(s.work_start.as<int>(), s.tile_start.as<int>(), ntiles, s.tile_start.as<int>() + ntiles, s);
     hipLaunchKernelGGL(k_tile_fill_block, dim3(ntiles), dim3(256), 0, s.stream, (const int*)s.cell_start.as<int>(),
                        (const int*)s.cell_count.as<int>(), (const int*)s.tile_start.as<int>(), tg,

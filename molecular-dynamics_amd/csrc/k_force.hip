@@ -263,6 +263,51 @@ __global__ void k_fix_to_force(unsigned long long* __restrict__ acc, int n, doub
     acc[3 * (size_t)i + 2] = 0ull;
 }
 
+// End of a DEM step: the force (from the fixed-point reaction accumulators
+// with FIX, which are cleared, else as stored by k_dem), the final kick
+// v += f dt/2 and, with ADV, the next step's kick and drift, with the
+// mirror kept current when inv is given (k_final_integration and
+// k_initial_integration arithmetic, in that order).
+template <int FIX, int ADV>
+__global__ void k_dem_finish(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force,
+                             double4* __restrict__ vel, double hdt, double dt, double4* __restrict__ pos,
+                             const int* __restrict__ inv, double4* __restrict__ spos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 f;
+    if (FIX) {
+        const long long ax = (long long)acc[3 * (size_t)i + 0];
+        const long long ay = (long long)acc[3 * (size_t)i + 1];
+        const long long az = (long long)acc[3 * (size_t)i + 2];
+        f = make_double4((double)ax * kFixInv, (double)ay * kFixInv, (double)az * kFixInv, 0.0);
+        force[i] = f;
+        acc[3 * (size_t)i + 0] = 0ull;
+        acc[3 * (size_t)i + 1] = 0ull;
+        acc[3 * (size_t)i + 2] = 0ull;
+    } else {
+        f = force[i];
+    }
+    double4 v = vel[i];
+    v.x = v.x + f.x * hdt;
+    v.y = v.y + f.y * hdt;
+    v.z = v.z + f.z * hdt;
+    if (ADV) {
+        v.x = v.x + f.x * hdt;
+        v.y = v.y + f.y * hdt;
+        v.z = v.z + f.z * hdt;
+        double4 x = pos[i];
+        x.x = x.x + v.x * dt;
+        x.y = x.y + v.y * dt;
+        x.z = x.z + v.z * dt;
+        pos[i] = x;
+        if (inv) {
+            const int k = inv[i];
+            if (k >= 0) spos[k] = x;
+        }
+    }
+    vel[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // energy and virial (always FP64): U = 4 eps (sr12 - sr6) [+ shift terms],
 // W = sum r.F; full lists weight each entry 1/2, half lists weight
@@ -541,7 +586,7 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
     if (half && s.sync_calls) check_fix_flag(s);
 }
 
-void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu) {
+void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double kick_dt, bool advance) {
     if (has_error()) return;
     TMD_REQUIRE(s.nb_valid, "compute forces: no neighbour list (call md_assemble_neighborlists first)");
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
@@ -568,7 +613,25 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
         if (half) hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
         else hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
     }
-    if (half) hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F);
+    if (kick_dt > 0.0) {
+        // final kick (and, advancing, the next kick and drift) fused
+        double4* V2 = s.vel.as<double4>();
+        double4* X = s.pos.as<double4>();
+        const int* I = s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr;
+        double4* M = s.spos.as<double4>();
+        const double hdt = 0.5 * kick_dt;
+        const dim3 g(div_up(s.n, 256)), b(256);
+        if (half) {
+            if (advance) hipLaunchKernelGGL((k_dem_finish<1, 1>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
+            else hipLaunchKernelGGL((k_dem_finish<1, 0>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
+        } else {
+            if (advance) hipLaunchKernelGGL((k_dem_finish<0, 1>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
+            else hipLaunchKernelGGL((k_dem_finish<0, 0>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
+        }
+        if (advance && !s.inv_valid) mirror_invalidate(s);
+    } else if (half) {
+        hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F);
+    }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
     if (half && s.sync_calls) check_fix_flag(s);

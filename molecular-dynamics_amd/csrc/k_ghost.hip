@@ -362,23 +362,31 @@ void refresh_ghosts(Sim& s) {
     TMD_HIP(hipGetLastError());
 }
 
-// pos[parts[k]] = mirror[k] for the locals of the cell CSR
+// pos[parts[k]] = mirror[k] for the locals of the cell CSR (when pos is
+// given), and likewise the velocities from their mirror order (when svel is)
 __global__ void k_unmirror(const double4* __restrict__ spos, const int* __restrict__ parts, int nbinned, int n,
-                           double4* __restrict__ pos) {
+                           double4* __restrict__ pos, const double4* __restrict__ svel, double4* __restrict__ vel) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nbinned) return;
     const int j = parts[k];
-    if (j < n) pos[j] = spos[k];
+    if (j < n) {
+        if (pos) pos[j] = spos[k];
+        if (svel) vel[j] = svel[k];
+    }
 }
 
+// Positions (and velocities) of the fused step loop back into atom order.
 void sync_pos_from_mirror(Sim& s) {
-    if (has_error() || !s.pos_stale) return;
+    if (has_error() || (!s.pos_stale && !s.vel_mirrored)) return;
     const int nt = 256;
     if (s.nbinned > 0)
         hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.spos.as<double4>(),
-                           (const int*)s.cell_parts.as<int>(), s.nbinned, s.n, s.pos.as<double4>());
+                           (const int*)s.cell_parts.as<int>(), s.nbinned, s.n,
+                           s.pos_stale ? s.pos.as<double4>() : (double4*)nullptr,
+                           s.vel_mirrored ? (const double4*)s.svel.as<double4>() : (const double4*)nullptr, s.vel.as<double4>());
     TMD_HIP(hipGetLastError());
     s.pos_stale = 0;
+    s.vel_mirrored = 0;
 }
 
 // velocities of the locals between atom order (vel) and mirror order (svel)

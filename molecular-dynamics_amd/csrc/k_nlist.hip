@@ -15,12 +15,15 @@
 
 namespace tmd {
 
-// Cell-ordered position mirror: spos[k] = pos[parts[k]] (locals and ghosts).
+// Cell-ordered position mirror: spos[k] = pos[parts[k]] (locals and ghosts);
+// with svel, also the locals' velocities in mirror order.
 __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict__ parts, const int* __restrict__ nbinned,
-                         double4* __restrict__ spos) {
+                         double4* __restrict__ spos, int n, const double4* __restrict__ vel, double4* __restrict__ svel) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *nbinned) return;
-    spos[k] = pos[parts[k]];
+    const int j = parts[k];
+    spos[k] = pos[j];
+    if (svel && j < n) svel[k] = vel[j];
 }
 
 template <bool HALF>
@@ -107,11 +110,19 @@ void build_windows(Sim& s) {
 void refresh_mirror(Sim& s) {
     if (has_error() || s.nbinned == 0) return;
     const int nt = 256;
+    // inside the fused step loop the velocities follow the new order too
+    const bool with_vel = s.vel_mirror_wanted && !s.vel_mirrored && s.n > 0;
+    if (with_vel) {
+        s.svel.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+        if (has_error()) return;
+    }
     hipLaunchKernelGGL(k_mirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
                        (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
-                       s.spos.as<double4>());
+                       s.spos.as<double4>(), s.n, (const double4*)s.vel.as<double4>(),
+                       with_vel ? s.svel.as<double4>() : (double4*)nullptr);
     TMD_HIP(hipGetLastError());
     s.mirror_fresh = 1;
+    if (with_vel) s.vel_mirrored = 1;
 }
 
 void build_neighbor_lists(Sim& s, int half, double cutoff) {

@@ -162,6 +162,7 @@ static void upload(Sim& s) {
     mirror_invalidate(s);
     s.pos_stale = 0;  // pos and vel are uploaded whole
     s.vel_mirrored = 0;
+    s.vel_mirror_wanted = 0;
     TMD_HIP(hipMemcpyAsync(s.pos.p, p.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.vel.p, v.data(), sizeof(double4) * nall, hipMemcpyHostToDevice, s.stream));
     TMD_HIP(hipMemcpyAsync(s.mass.p, s.h_mass.data(), sizeof(double) * nall, hipMemcpyHostToDevice, s.stream));
@@ -307,6 +308,7 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     mirror_invalidate(s);
     s.pos_stale = 0;
     s.vel_mirrored = 0;
+    s.vel_mirror_wanted = 0;
     s.h_mass.assign(masses, masses + nparticles);
     s.h_pos.resize(3 * (size_t)nparticles);
     s.h_vel.resize(3 * (size_t)nparticles);
@@ -452,6 +454,7 @@ void md_deallocate_grid() {
     mirror_invalidate(s);
     s.pos_stale = 0;
     s.vel_mirrored = 0;
+    s.vel_mirror_wanted = 0;
     s.nbinned = 0;
     s.dev_valid = false;
 }
@@ -778,12 +781,11 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         // refresh (or a rebuild) and one force pass
         if (steps > 0) initial_integration(s, dt);
         if (!hf) mirror_velocities(s, true);
+        s.vel_mirror_wanted = hf ? 0 : 1;  // the rebuild's mirror pass moves them again
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {
                 sync_pos_from_mirror(s);  // the rebuild reads pos and vel
-                mirror_velocities(s, false);
                 rebuild(s, hf, cutoff + skin);
-                if (!hf) mirror_velocities(s, true);
             } else {
                 hipEvent_t ev = nullptr;
                 s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
@@ -793,8 +795,24 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
             compute_lj(s, hf, cutoff, p0, p1, dt, j + 1 < steps);
             ++done;
         }
+        s.vel_mirror_wanted = 0;
         sync_pos_from_mirror(s);
-        mirror_velocities(s, false);
+    } else if (dem) {
+        // DEM: the step's end (fixed-point reactions to forces, final kick)
+        // and the next step's kick and drift are one pass after the force
+        if (steps > 0) initial_integration(s, dt);
+        for (int j = 0; j < steps && !has_error(); ++j) {
+            if ((j + 1) % reneigh_every == 0) {
+                rebuild(s, hf, cutoff + skin);
+            } else {
+                hipEvent_t ev = nullptr;
+                s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
+                refresh_ghosts(s);
+                s.timer.end(K_GHOST_SYNC, s.stream, ev);
+            }
+            compute_dem(s, hf, cutoff, p0, p1, p2, p3, dt, j + 1 < steps);
+            ++done;
+        }
     } else {
         for (int j = 0; j < steps && !has_error(); ++j) {
             initial_integration(s, dt);
@@ -806,12 +824,7 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
                 refresh_ghosts(s);
                 s.timer.end(K_GHOST_SYNC, s.stream, ev);
             }
-            if (dem) {
-                compute_dem(s, hf, cutoff, p0, p1, p2, p3);
-                final_integration(s, dt);
-            } else {
-                compute_lj(s, hf, cutoff, p0, p1, dt);  // final kick fused
-            }
+            compute_lj(s, hf, cutoff, p0, p1, dt);  // final kick fused
             ++done;
         }
     }

@@ -104,6 +104,7 @@ struct Sim {
                             // mirror) still hold the previous step
     int pos_stale = 0;      // local positions in pos lag the mirror (fused step loop; sync_pos_from_mirror)
     int vel_mirrored = 0;   // local velocities live in svel, in mirror order (fused step loop)
+    int vel_mirror_wanted = 0;  // fused step loop: refresh_mirror also moves the velocities to svel
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
@@ -199,7 +200,10 @@ void build_neighbor_lists(Sim& s, int half, double cutoff);
 // force pass; with advance also the next step's kick and drift (positions
 // move; the mirror must be refreshed before the next force evaluation)
 void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt = 0.0, bool advance = false);
-void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu);
+// kick_dt > 0: the final kick of the step fused; advance: also the next
+// step's first kick and drift
+void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double kick_dt = 0.0,
+                 bool advance = false);
 void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs);
 
 void reduce_partials(Sim& s, const double* partials, int nblocks, int width, double* out);
