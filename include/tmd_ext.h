@@ -12,7 +12,7 @@ extern "C" {
 
 /* ---- errors ------------------------------------------------------------- */
 /* First recorded error (HIP failure, invalid geometry, escaped atoms,
- * fixed-point overflow), or NULL.  The reference has no error channel. */
+ * 16-bit window overflow), or NULL.  The reference has no error channel. */
 const char* tmd_last_error(void);
 void tmd_clear_error(void);
 /* Device name into buf; returns the number of compute units (0 = no GPU). */

@@ -455,8 +455,11 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __re
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = staged ? nlist_atom<false, true>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list)
-                               : nlist_atom<false, false>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list);
+        int unused = 0;
+        const int cnt = staged ? nlist_atom<false, true>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list,
+                                                         nullptr, unused)
+                               : nlist_atom<false, false>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list,
+                                                          nullptr, unused);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(&flags[FL_LIST], cnt);
     }

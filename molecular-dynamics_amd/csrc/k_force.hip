@@ -11,12 +11,13 @@
 // Full lists accumulate the force of lane i in registers and store it once:
 // no atomics (core/compute.impala:92-123, full branch).
 //
-// Half lists (each pair once) use a deterministic reduction: the force on i
-// is summed in registers and every reaction -F on a local j is added to a
-// 64-bit fixed-point accumulator (scale 2^40) with integer atomics.  Integer
-// addition is associative, so the result does not depend on the order in
-// which waves run (the reference's half-list "+=" is a data race on GPUs,
-// core/compute.impala:104-111).
+// Half lists (each pair once in the list) are evaluated by gather as well:
+// the list build also writes each atom's complementary column (its local
+// neighbours j > i), and the atom sums its half-list terms and its
+// complementary terms.  A complementary term is, bit for bit, the negated
+// reaction the reference scatters to i with "+=" (a data race on GPUs,
+// core/compute.impala:104-111), so the result needs no atomics and does not
+// depend on the order in which waves run.
 //
 // FP64 LJ follows the reference closure operation by operation
 // (core/runtime.impala:188-201) in list order, so full-list FP64 forces are
@@ -49,11 +50,18 @@ namespace tmd {
 // step that is followed by another one is consumed here (the kick) and the
 // loop restores pos from the mirror before it rebuilds or returns.  The
 // arithmetic is that of the separate integration kernels.
-template <int PREC, int SHIFT, int UNIT, int KICK>
+//
+// PAIR2 (half lists): the atom also gathers its complementary column (local
+// neighbours j > i, whose pair terms the half list evaluates from j's side);
+// the two partial sums are added.  Each term equals, bit for bit, the
+// negated reaction the reference scatters to i (dx and dv change sign
+// exactly), so only the summation order differs from the reference's.
+template <int PREC, int SHIFT, int UNIT, int KICK, bool PAIR2>
 __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
                                                          double4* __restrict__ vel, double hdt, double dt,
                                                          const int* __restrict__ wk, double4* __restrict__ spos_next,
-                                                         int vel_by_k) {
+                                                         int vel_by_k, const int* __restrict__ nbc2,
+                                                         const nb_t* __restrict__ nbl2) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -69,8 +77,16 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
         const double4 pi = spos[kw];
         const int cnt = nbc[w];
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
-        const double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
-                                 : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
+        double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
+                           : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
+        if (PAIR2) {
+            const int4* lp2 = reinterpret_cast<const int4*>(nbl2 + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+            const double4 f2 = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, nbc2[w], lp2, sp, spos, W, a)
+                                      : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, nbc2[w], lp2, sp, spos, W, a);
+            f.x = f.x + f2.x;
+            f.y = f.y + f2.y;
+            f.z = f.z + f2.z;
+        }
         if (KICK != 2) force[i] = f;
         if (KICK) {
             // velocity-Verlet final kick (k_final_integration's arithmetic)
@@ -95,198 +111,15 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
     }
 }
 
-__device__ __forceinline__ unsigned long long to_fix(double v) { return (unsigned long long)__double2ll_rn(v * kFixScale); }
-
-// Half-list LJ.  Each pair is evaluated once, by the lane of its higher
-// index (local-ghost pairs by the local side; no reaction on ghosts).  The
-// force on the lane's own atom is summed in registers in list order and
-// stored once (`own`).  Reactions -F on local neighbours are added, as
-// 64-bit fixed-point integers (scale 2^40), to LDS accumulators of the
-// staged window, and each window atom's accumulated reaction is added to
-// its global accumulator with one integer atomic per component when the
-// block ends.  Integer addition is associative, so the result does not
-// depend on the order in which lanes and blocks run.  A window too large
-// for LDS sends reactions straight to the global accumulators.
-template <int PREC, int SHIFT>
-__global__ void __launch_bounds__(kMaxTileThreads) k_lj_half(TMD_KERNEL_LIST_PARAMS, LJArgs a, int wcap_half,
-                                                         double4* __restrict__ own, unsigned long long* __restrict__ acc,
-                                                         int* __restrict__ flag) {
-    extern __shared__ double dyn[];
-    const int tile = xcd_block(blockIdx.x, gridDim.x);
-    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
-    if (nloc == 0) return;
-    __shared__ TileWin W;
-    tile_window(tile, start, a.g, W);
-    const WinStage sp = win_stage(dyn, wcap_half);
-    // wcap_half is even, so the accumulators start 8-byte aligned
-    unsigned long long* racc = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(dyn) + (size_t)kStageBytes * wcap_half);
-    const bool staged = stage_window(W, wcap_half, spos, parts, sp);
-    const int S = win_size(W);
-    if (staged) {
-        for (int t = threadIdx.x; t < 3 * S; t += blockDim.x) racc[t] = 0ull;
-        __syncthreads();
-    }
-    bool big = false;
-    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
-        const int w = w0 + at;
-        const int i = work[w];
-        const double4 pi = pos[i];
-        const int cnt = nbc[w];
-        const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        const int ng = (cnt + 7) >> 3;
-        int4 q = lp[0];  // group 0 of the column always exists (capacity >= 8)
-        for (int gi = 0; gi < ng; ++gi) {
-            const int4 e = q;
-            q = lp[(size_t)min(gi + 1, ng - 1) * 64];  // next group (clamped: always a valid load)
-            const int k0 = gi * 8;
-            double dx[8], dy[8], dz[8], rs[8];
-            int jj[8], tt[8];
-            bool ok[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const bool valid = k0 + u < cnt;
-                const int t = valid ? nb_entry(e, u) : 0;
-                double xj, yj, zj;
-                if (staged) {
-                    xj = sp.xyz[3 * t + 0]; yj = sp.xyz[3 * t + 1]; zj = sp.xyz[3 * t + 2];
-                    jj[u] = sp.j[t];
-                } else {
-                    const int kk = win_index(t, W);
-                    const double4 r = spos[kk];
-                    xj = r.x; yj = r.y; zj = r.z;
-                    jj[u] = parts[kk];
-                }
-                tt[u] = t;
-                dx[u] = pi.x - xj;
-                dy[u] = pi.y - yj;
-                dz[u] = pi.z - zj;
-                const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-                ok[u] = valid && rsq < a.rc2;
-                rs[u] = ok[u] ? rsq : 1.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                double fm;
-                if (PREC == PREC_FP64) {
-                    const double f = lj_f64u<SHIFT, 0>(a, rs[u]);
-                    fm = ok[u] ? f : 0.0;
-                } else {
-                    const float f = lj_f32u<SHIFT, 0>(a, (float)rs[u]);
-                    fm = (double)(ok[u] ? f : 0.0f);
-                }
-                const double Fx = dx[u] * fm, Fy = dy[u] * fm, Fz = dz[u] * fm;
-                fx = fx + Fx;
-                fy = fy + Fy;
-                fz = fz + Fz;
-                if (ok[u] && jj[u] < a.n) {
-                    big |= fabs(Fx) > kFixMax || fabs(Fy) > kFixMax || fabs(Fz) > kFixMax;
-                    const unsigned long long rx = to_fix(-Fx), ry = to_fix(-Fy), rz = to_fix(-Fz);
-                    if (staged) {
-                        atomicAdd(&racc[3 * tt[u] + 0], rx);
-                        atomicAdd(&racc[3 * tt[u] + 1], ry);
-                        atomicAdd(&racc[3 * tt[u] + 2], rz);
-                    } else {
-                        atomicAdd(&acc[3 * (size_t)jj[u] + 0], rx);
-                        atomicAdd(&acc[3 * (size_t)jj[u] + 1], ry);
-                        atomicAdd(&acc[3 * (size_t)jj[u] + 2], rz);
-                    }
-                }
-            }
-        }
-        own[i] = make_double4(fx, fy, fz, 0.0);
-    }
-    if (big) atomicOr(flag, 1);
-    if (staged) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < S; t += blockDim.x) {
-            const int j = sp.j[t];
-            if (j < a.n) {
-                const unsigned long long rx = racc[3 * t + 0], ry = racc[3 * t + 1], rz = racc[3 * t + 2];
-                if ((rx | ry | rz) != 0ull) {
-                    atomicAdd(&acc[3 * (size_t)j + 0], rx);
-                    atomicAdd(&acc[3 * (size_t)j + 1], ry);
-                    atomicAdd(&acc[3 * (size_t)j + 2], rz);
-                }
-            }
-        }
-    }
-}
-
-// force = own sum + accumulated reactions; clears the accumulators; KICK
-// as in k_lj_full
-template <int KICK>
-__global__ void k_half_finish(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force,
-                              double4* __restrict__ vel, double hdt, double dt, double4* __restrict__ pos) {
+// End of a DEM step: the final kick v += f dt/2 and, with ADV, the next
+// step's kick and drift, with the mirror kept current when inv is given
+// (k_final_integration and k_initial_integration arithmetic, in that order).
+template <int ADV>
+__global__ void k_dem_finish(int n, const double4* __restrict__ force, double4* __restrict__ vel, double hdt, double dt,
+                             double4* __restrict__ pos, const int* __restrict__ inv, double4* __restrict__ spos) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const long long ax = (long long)acc[3 * (size_t)i + 0];
-    const long long ay = (long long)acc[3 * (size_t)i + 1];
-    const long long az = (long long)acc[3 * (size_t)i + 2];
-    double4 f = force[i];
-    f.x = f.x + (double)ax * kFixInv;
-    f.y = f.y + (double)ay * kFixInv;
-    f.z = f.z + (double)az * kFixInv;
-    force[i] = f;
-    acc[3 * (size_t)i + 0] = 0ull;
-    acc[3 * (size_t)i + 1] = 0ull;
-    acc[3 * (size_t)i + 2] = 0ull;
-    if (KICK) {
-        double4 v = vel[i];
-        v.x = v.x + f.x * hdt;
-        v.y = v.y + f.y * hdt;
-        v.z = v.z + f.z * hdt;
-        if (KICK == 2) {
-            v.x = v.x + f.x * hdt;
-            v.y = v.y + f.y * hdt;
-            v.z = v.z + f.z * hdt;
-            double4 x = pos[i];
-            x.x = x.x + v.x * dt;
-            x.y = x.y + v.y * dt;
-            x.z = x.z + v.z * dt;
-            pos[i] = x;
-        }
-        vel[i] = v;
-    }
-}
-
-// fixed-point accumulators -> force (DEM half lists)
-__global__ void k_fix_to_force(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long long ax = (long long)acc[3 * (size_t)i + 0];
-    const long long ay = (long long)acc[3 * (size_t)i + 1];
-    const long long az = (long long)acc[3 * (size_t)i + 2];
-    force[i] = make_double4((double)ax * kFixInv, (double)ay * kFixInv, (double)az * kFixInv, 0.0);
-    acc[3 * (size_t)i + 0] = 0ull;
-    acc[3 * (size_t)i + 1] = 0ull;
-    acc[3 * (size_t)i + 2] = 0ull;
-}
-
-// End of a DEM step: the force (from the fixed-point reaction accumulators
-// with FIX, which are cleared, else as stored by k_dem), the final kick
-// v += f dt/2 and, with ADV, the next step's kick and drift, with the
-// mirror kept current when inv is given (k_final_integration and
-// k_initial_integration arithmetic, in that order).
-template <int FIX, int ADV>
-__global__ void k_dem_finish(unsigned long long* __restrict__ acc, int n, double4* __restrict__ force,
-                             double4* __restrict__ vel, double hdt, double dt, double4* __restrict__ pos,
-                             const int* __restrict__ inv, double4* __restrict__ spos) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 f;
-    if (FIX) {
-        const long long ax = (long long)acc[3 * (size_t)i + 0];
-        const long long ay = (long long)acc[3 * (size_t)i + 1];
-        const long long az = (long long)acc[3 * (size_t)i + 2];
-        f = make_double4((double)ax * kFixInv, (double)ay * kFixInv, (double)az * kFixInv, 0.0);
-        force[i] = f;
-        acc[3 * (size_t)i + 0] = 0ull;
-        acc[3 * (size_t)i + 1] = 0ull;
-        acc[3 * (size_t)i + 2] = 0ull;
-    } else {
-        f = force[i];
-    }
+    const double4 f = force[i];
     double4 v = vel[i];
     v.x = v.x + f.x * hdt;
     v.y = v.y + f.y * hdt;
@@ -394,10 +227,14 @@ __device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R r
     }
 }
 
-template <int PREC, int HALF>
+// DEM forces by gather.  PAIR2 (half lists): the atom also gathers its
+// complementary column (local neighbours j > i); each such term is, bit for
+// bit, the negated reaction the half list would send to i (dx and dv change
+// sign exactly), so no scatter is needed.
+template <int PREC, bool PAIR2>
 __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __restrict__ vel, DEMArgs a,
-                                                     double4* __restrict__ force, unsigned long long* __restrict__ acc,
-                                                     int* __restrict__ flag) {
+                                                     double4* __restrict__ force, const int* __restrict__ nbc2,
+                                                     const nb_t* __restrict__ nbl2) {
     extern __shared__ double dyn[];
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
@@ -411,82 +248,54 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS,
     const bool staged = stage_tile(W, a.wcap, spos, parts, sx, sy, sz, sj);
     // STAGED as a compile-time constant: LDS reads, not generic-pointer loads
     auto atoms = [&](auto staged_c) {
-    constexpr bool STAGED = decltype(staged_c)::value;
-    for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
-    const int w = w0 + at;
-    const int i = work[w];
-    const double4 pi = pos[i];
-    const double4 vi = vel[i];
-    const int cnt = nbc[w];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    bool big = false;
-    for (int k = 0; k < cnt; ++k) {
-        const int t = nbl[nb16_slot(w, k, a.cap)];
-        int j;
-        double xj, yj, zj;
-        if (STAGED) {
-            j = sj[t];
-            xj = sx[t]; yj = sy[t]; zj = sz[t];
-        } else {
-            const int kk = win_index(t, W);
-            j = parts[kk];
-            const double4 p = spos[kk];
-            xj = p.x; yj = p.y; zj = p.z;
+        constexpr bool STAGED = decltype(staged_c)::value;
+        for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+            const int w = w0 + at;
+            const int i = work[w];
+            const double4 pi = pos[i];
+            const double4 vi = vel[i];
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            auto entries = [&](const nb_t* __restrict__ L, int cnt) {
+                for (int k = 0; k < cnt; ++k) {
+                    const int t = L[nb16_slot(w, k, a.cap)];
+                    int j;
+                    double xj, yj, zj;
+                    if (STAGED) {
+                        j = sj[t];
+                        xj = sx[t]; yj = sy[t]; zj = sz[t];
+                    } else {
+                        const int kk = win_index(t, W);
+                        j = parts[kk];
+                        const double4 p = spos[kk];
+                        xj = p.x; yj = p.y; zj = p.z;
+                    }
+                    const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                    const double rsq = dx * dx + dy * dy + dz * dz;
+                    if (rsq < a.rc2) {
+                        const double4 vj = vel[j];
+                        double Fx, Fy, Fz;
+                        if (PREC == PREC_FP64) {
+                            dem_pair<double>(a, dx, dy, dz, rsq, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, Fx, Fy, Fz);
+                        } else {
+                            float gx, gy, gz;
+                            dem_pair<float>(a, (float)dx, (float)dy, (float)dz, (float)rsq, (float)(vi.x - vj.x),
+                                            (float)(vi.y - vj.y), (float)(vi.z - vj.z), gx, gy, gz);
+                            Fx = gx; Fy = gy; Fz = gz;
+                        }
+                        fx = fx + Fx; fy = fy + Fy; fz = fz + Fz;
+                    }
+                }
+            };
+            entries(nbl, nbc[w]);
+            if (PAIR2) entries(nbl2, nbc2[w]);
+            force[i] = make_double4(fx, fy, fz, 0.0);
         }
-        const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        if (rsq < a.rc2) {
-            const double4 vj = vel[j];
-            double Fx, Fy, Fz;
-            if (PREC == PREC_FP64) {
-                dem_pair<double>(a, dx, dy, dz, rsq, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, Fx, Fy, Fz);
-            } else {
-                float gx, gy, gz;
-                dem_pair<float>(a, (float)dx, (float)dy, (float)dz, (float)rsq, (float)(vi.x - vj.x), (float)(vi.y - vj.y),
-                                (float)(vi.z - vj.z), gx, gy, gz);
-                Fx = gx; Fy = gy; Fz = gz;
-            }
-            fx = fx + Fx; fy = fy + Fy; fz = fz + Fz;
-            if (HALF && j < a.n) {
-                big |= fabs(Fx) > kFixMax || fabs(Fy) > kFixMax || fabs(Fz) > kFixMax;
-                atomicAdd(&acc[3 * (size_t)j + 0], to_fix(-Fx));
-                atomicAdd(&acc[3 * (size_t)j + 1], to_fix(-Fy));
-                atomicAdd(&acc[3 * (size_t)j + 2], to_fix(-Fz));
-            }
-        }
-    }
-    if (HALF) {
-        big |= fabs(fx) > kFixMax || fabs(fy) > kFixMax || fabs(fz) > kFixMax;
-        atomicAdd(&acc[3 * (size_t)i + 0], to_fix(fx));
-        atomicAdd(&acc[3 * (size_t)i + 1], to_fix(fy));
-        atomicAdd(&acc[3 * (size_t)i + 2], to_fix(fz));
-        if (big) atomicOr(flag, 1);
-    } else {
-        force[i] = make_double4(fx, fy, fz, 0.0);
-    }
-    }
     };
     if (staged) atoms(std::true_type{});
     else atoms(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
-
-static void ensure_acc(Sim& s) {
-    const size_t need = sizeof(unsigned long long) * 3 * (size_t)(s.n > 0 ? s.n : 1);
-    if (s.acc.bytes < need) {
-        s.acc.reserve(need, false, s.stream);
-        TMD_HIP(hipMemsetAsync(s.acc.p, 0, s.acc.bytes, s.stream));
-    }
-}
-
-static void check_fix_flag(Sim& s) {
-    int host = 0;
-    int* flag = s.counters.as<int>() + 4;
-    TMD_HIP(hipMemcpyAsync(&host, flag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    TMD_HIP(hipStreamSynchronize(s.stream));
-    if (host) set_error(__FILE__, __LINE__, "half-list force exceeds the fixed-point accumulator range (|F| > 4e6)");
-}
 
 #define TMD_LIST_ARGS                                                                                              \
     (const double4*)s.pos.as<double4>(), (const double4*)s.spos.as<double4>(), (const int*)s.cell_start.as<int>(), \
@@ -508,7 +317,7 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
     if (!s.mirror_fresh) refresh_mirror(s);
     s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
-    if (!half) {
+    {
         double4* F = s.force.as<double4>();
         const size_t lds = lds_bytes(s, false);
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
@@ -518,17 +327,18 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         const int VK = s.vel_mirrored;
         const int* WK = s.work_k.as<int>();
         double4* SN = s.spos2.as<double4>();
+        const int* C2 = s.nb_count2.as<int>();
+        const nb_t* L2 = s.nb_list2.as<nb_t>();
+#define TMD_LJ_K(P, SH, U, K, H)                                                                                       \
+    hipLaunchKernelGGL((k_lj_full<P, SH, U, K, H>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
+                       kick_dt, WK, SN, VK, C2, L2)
+#define TMD_LJ_H(P, SH, U, K)                                                                                          \
+    do { if (half) TMD_LJ_K(P, SH, U, K, true); else TMD_LJ_K(P, SH, U, K, false); } while (0)
 #define TMD_LJ_FULL(P, SH, U)                                                                                          \
     do {                                                                                                               \
-        if (kick_dt > 0.0 && advance)                                                                                  \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 2>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, WK, SN, VK);                                                                    \
-        else if (kick_dt > 0.0)                                                                                        \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, WK, SN, VK);                                                                    \
-        else                                                                                                           \
-            hipLaunchKernelGGL((k_lj_full<P, SH, U, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
-                               kick_dt, WK, SN, VK);                                                                    \
+        if (kick_dt > 0.0 && advance) TMD_LJ_H(P, SH, U, 2);                                                           \
+        else if (kick_dt > 0.0) TMD_LJ_H(P, SH, U, 1);                                                                 \
+        else TMD_LJ_H(P, SH, U, 0);                                                                                    \
     } while (0)
         if (s.precision == PREC_FP64) {
             if (shift) { if (unit) TMD_LJ_FULL(PREC_FP64, 1, 1); else TMD_LJ_FULL(PREC_FP64, 1, 0); }
@@ -541,49 +351,19 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
             else { if (unit) TMD_LJ_FULL(PREC_FP32_KAHAN, 0, 1); else TMD_LJ_FULL(PREC_FP32_KAHAN, 0, 0); }
         }
 #undef TMD_LJ_FULL
-    } else {
-        ensure_acc(s);
-        int* flag = s.counters.as<int>() + 4;
-        TMD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s.stream));
-        unsigned long long* A = s.acc.as<unsigned long long>();
-        // window records (32 B) + three 64-bit reaction accumulators per atom
-        const int per = kStageBytes + 3 * (int)sizeof(unsigned long long);
-        const int wh = std::min(s.max_window + 2, std::min(s.lds_limit, kLdsHalfBudget) / per) & ~1;  // + sentinel slot
-        const size_t lds = (size_t)wh * per;
-        double4* F = s.force.as<double4>();
-#define TMD_LJ_HALF(P, SH) \
-    hipLaunchKernelGGL((k_lj_half<P, SH>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, wh, F, A, flag)
-        if (s.precision == PREC_FP64) {
-            if (shift) TMD_LJ_HALF(PREC_FP64, 1); else TMD_LJ_HALF(PREC_FP64, 0);
-        } else {
-            if (shift) TMD_LJ_HALF(PREC_FP32_MIXED, 1); else TMD_LJ_HALF(PREC_FP32_MIXED, 0);
-        }
-#undef TMD_LJ_HALF
-        double4* V = s.vel.as<double4>();
-        double4* X = s.pos.as<double4>();
-        const int nb256 = div_up(s.n, 256);
-        if (kick_dt > 0.0 && advance)
-            hipLaunchKernelGGL(k_half_finish<2>, dim3(nb256), dim3(256), 0, s.stream, A, s.n, F, V, 0.5 * kick_dt, kick_dt, X);
-        else if (kick_dt > 0.0)
-            hipLaunchKernelGGL(k_half_finish<1>, dim3(nb256), dim3(256), 0, s.stream, A, s.n, F, V, 0.5 * kick_dt, kick_dt, X);
-        else
-            hipLaunchKernelGGL(k_half_finish<0>, dim3(nb256), dim3(256), 0, s.stream, A, s.n, F, V, 0.0, 0.0, X);
+#undef TMD_LJ_H
+#undef TMD_LJ_K
     }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
     if (kick_dt > 0.0 && advance) {
-        if (!half) {
-            // the new local positions are in spos2: it becomes the mirror, whose
-            // ghost entries step_refresh rewrites
-            std::swap(s.spos, s.spos2);
-            s.mirror_fresh = 0;
-            s.ghost_stale = 1;
-            s.pos_stale = 1;
-        } else {
-            mirror_invalidate(s);  // local positions advanced
-        }
+        // the new local positions are in spos2: it becomes the mirror, whose
+        // ghost entries step_refresh rewrites
+        std::swap(s.spos, s.spos2);
+        s.mirror_fresh = 0;
+        s.ghost_stale = 1;
+        s.pos_stale = 1;
     }
-    if (half && s.sync_calls) check_fix_flag(s);
 }
 
 void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double kick_dt, bool advance) {
@@ -594,25 +374,23 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     const double radius = s.dem_radius > 0.0 ? s.dem_radius : 0.5 * rc;
     DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n, window_cap(s), s.tiles};
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
-    ensure_acc(s);
-    int* flag = s.counters.as<int>() + 4;
-    TMD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s.stream));
     const double4* V = s.vel.as<double4>();
     double4* F = s.force.as<double4>();
-    unsigned long long* A = s.acc.as<unsigned long long>();
+    const int* C2 = s.nb_count2.as<int>();
+    const nb_t* L2 = s.nb_list2.as<nb_t>();
     const size_t lds = lds_bytes(s, true);
     hipEvent_t ev = nullptr;
     s.timer.begin(K_MIRROR, s.stream, &ev);
     if (!s.mirror_fresh) refresh_mirror(s);
     s.timer.end(K_MIRROR, s.stream, ev);
     s.timer.begin(K_FORCE, s.stream, &ev);
+#define TMD_DEM(P, H) hipLaunchKernelGGL((k_dem<P, H>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, C2, L2)
     if (s.precision == PREC_FP64) {
-        if (half) hipLaunchKernelGGL((k_dem<PREC_FP64, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
-        else hipLaunchKernelGGL((k_dem<PREC_FP64, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        if (half) TMD_DEM(PREC_FP64, true); else TMD_DEM(PREC_FP64, false);
     } else {
-        if (half) hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 1>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
-        else hipLaunchKernelGGL((k_dem<PREC_FP32_MIXED, 0>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, V, a, F, A, flag);
+        if (half) TMD_DEM(PREC_FP32_MIXED, true); else TMD_DEM(PREC_FP32_MIXED, false);
     }
+#undef TMD_DEM
     if (kick_dt > 0.0) {
         // final kick (and, advancing, the next kick and drift) fused
         double4* V2 = s.vel.as<double4>();
@@ -621,20 +399,12 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
         double4* M = s.spos.as<double4>();
         const double hdt = 0.5 * kick_dt;
         const dim3 g(div_up(s.n, 256)), b(256);
-        if (half) {
-            if (advance) hipLaunchKernelGGL((k_dem_finish<1, 1>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
-            else hipLaunchKernelGGL((k_dem_finish<1, 0>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
-        } else {
-            if (advance) hipLaunchKernelGGL((k_dem_finish<0, 1>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
-            else hipLaunchKernelGGL((k_dem_finish<0, 0>), g, b, 0, s.stream, A, s.n, F, V2, hdt, kick_dt, X, I, M);
-        }
+        if (advance) hipLaunchKernelGGL(k_dem_finish<1>, g, b, 0, s.stream, s.n, (const double4*)F, V2, hdt, kick_dt, X, I, M);
+        else hipLaunchKernelGGL(k_dem_finish<0>, g, b, 0, s.stream, s.n, (const double4*)F, V2, hdt, kick_dt, X, I, M);
         if (advance && !s.inv_valid) mirror_invalidate(s);
-    } else if (half) {
-        hipLaunchKernelGGL(k_fix_to_force, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, A, s.n, F);
     }
     s.timer.end(K_FORCE, s.stream, ev);
     TMD_HIP(hipGetLastError());
-    if (half && s.sync_calls) check_fix_flag(s);
 }
 
 void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part) {

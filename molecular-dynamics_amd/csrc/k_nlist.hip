@@ -32,6 +32,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __rest
                                                         const int* __restrict__ start, const int* __restrict__ parts,
                                                         const int* __restrict__ work, const int* __restrict__ tile_start,
                                                         NbArgs a, int* __restrict__ nb_count, nb_t* __restrict__ nb_list,
+                                                        int* __restrict__ nb_count2, nb_t* __restrict__ nb_list2,
                                                         int* __restrict__ overflow) {
     extern __shared__ double dyn[];
     int* cst = reinterpret_cast<int*>(reinterpret_cast<char*>(dyn) + (size_t)kStageBytes * a.wcap);
@@ -46,10 +47,15 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __rest
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
-        const int cnt = staged ? nlist_atom<HALF, true>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list)
-                               : nlist_atom<HALF, false>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list);
+        int cnt2 = 0;
+        const int cnt = staged ? nlist_atom<HALF, true>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list,
+                                                        nb_list2, cnt2)
+                               : nlist_atom<HALF, false>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list,
+                                                         nb_list2, cnt2);
         nb_count[w] = cnt;
-        if (cnt > a.cap) atomicMax(overflow, cnt);
+        if (HALF) nb_count2[w] = cnt2;
+        const int most = max(cnt, cnt2);
+        if (most > a.cap) atomicMax(overflow, most);
     }
 }
 
@@ -139,6 +145,11 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         s.nb_stride = ntile64 * 64;
         s.nb_count.reserve(sizeof(int) * (size_t)s.nb_stride, false, s.stream);
         s.nb_list.reserve(sizeof(nb_t) * (size_t)ntile64 * 64 * (size_t)s.nb_cap, false, s.stream);
+        if (half) {
+            // the complementary column of every half list (same layout)
+            s.nb_count2.reserve(sizeof(int) * (size_t)s.nb_stride, false, s.stream);
+            s.nb_list2.reserve(sizeof(nb_t) * (size_t)ntile64 * 64 * (size_t)s.nb_cap, false, s.stream);
+        }
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
         const int rsb = row_starts_bytes(s.tiles);
@@ -150,7 +161,8 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
     hipLaunchKernelGGL(k_nlist<H>, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(), \
                        (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(), (const int*)s.cell_inv.as<int>(), \
                        (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), \
-                       (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(), s.nb_list.as<nb_t>(), counters + 2)
+                       (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(), s.nb_list.as<nb_t>(), \
+                       s.nb_count2.as<int>(), s.nb_list2.as<nb_t>(), counters + 2)
             if (half) TMD_NLIST(true); else TMD_NLIST(false);
 #undef TMD_NLIST
         }

@@ -323,7 +323,6 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     for (int d = 0; d < 3; ++d) shell *= 1.0 + 2.0 * cell_spacing / s.g.len[d];
     const long long est = (long long)nparticles + (long long)(nparticles * (shell - 1.0) * 1.25) + 4096;
     alloc_atoms(s, (int)std::min<long long>(est, 0x7fffffffLL));
-    s.acc.release();
     return has_error() ? 0 : nparticles;
 }
 
@@ -441,8 +440,8 @@ void md_deallocate_grid() {
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
                       &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel,
-                      &s.nb_count, &s.nb_list,
-                      &s.acc, &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
+                      &s.nb_count, &s.nb_list, &s.nb_count2, &s.nb_list2,
+                      &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
                       &s.adj_acc};
     for (DevBuf* b : bufs) b->release();
@@ -780,8 +779,8 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         // the first kick + drift of the next one, so a step is a ghost/mirror
         // refresh (or a rebuild) and one force pass
         if (steps > 0) initial_integration(s, dt);
-        if (!hf) mirror_velocities(s, true);
-        s.vel_mirror_wanted = hf ? 0 : 1;  // the rebuild's mirror pass moves them again
+        mirror_velocities(s, true);
+        s.vel_mirror_wanted = 1;  // the rebuild's mirror pass moves them again
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {
                 sync_pos_from_mirror(s);  // the rebuild reads pos and vel
@@ -798,8 +797,8 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         s.vel_mirror_wanted = 0;
         sync_pos_from_mirror(s);
     } else if (dem) {
-        // DEM: the step's end (fixed-point reactions to forces, final kick)
-        // and the next step's kick and drift are one pass after the force
+        // DEM: the step's final kick and the next step's kick and drift are
+        // one pass after the force
         if (steps > 0) initial_integration(s, dt);
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {
@@ -830,11 +829,6 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
     }
     s.sync_calls = saved_sync;
     if (!has_error()) TMD_HIP(hipStreamSynchronize(s.stream));
-    if (hf && !has_error()) {
-        int host = 0;
-        TMD_HIP(hipMemcpy(&host, s.counters.as<int>() + 4, sizeof(int), hipMemcpyDeviceToHost));
-        if (host) set_error(__FILE__, __LINE__, "half-list force exceeds the fixed-point accumulator range (|F| > 4e6)");
-    }
     s.integration_iters = s.n;
     return has_error() ? 0 : done;
 }

@@ -40,12 +40,6 @@ enum Precision { PREC_FP64 = 0, PREC_FP32_MIXED = 1, PREC_FP32_KAHAN = 2 };
 enum HalfMode { HALF_GHOST_FULL = 0, HALF_REF_LITERAL = 1 };
 enum GhostMode { GHOST_FRESH = 0, GHOST_REF_LAG = 1 };
 
-// Fixed-point scale of the half-list force accumulators (2^40): forces up to
-// 8.4e6 in magnitude, resolution 9.1e-13.
-constexpr double kFixScale = 1099511627776.0;
-constexpr double kFixInv = 1.0 / 1099511627776.0;
-constexpr double kFixMax = 4.0e6;
-
 // Geometry of one periodic box (single rank: rank AABB == world AABB).
 struct Geom {
     double world[6];  // xmin xmax ymin ymax zmin zmax
@@ -134,7 +128,7 @@ struct Sim {
     DevBuf svel;                       // local velocities in mirror order during the fused step loop
     DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
     DevBuf nb_count, nb_list;
-    DevBuf acc;                        // int64 fixed-point force accumulators (half lists)
+    DevBuf nb_count2, nb_list2;        // half lists: the complementary columns (local j > i)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
     // trajectory loss and adjoint: initial state, checkpoints, interval
     // states, adjoints (double4 per local atom), partial sums, accumulators

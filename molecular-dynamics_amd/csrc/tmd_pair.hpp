@@ -227,6 +227,39 @@ struct NbArgs {
     TileGeom g;
 };
 
+// One list column being written: groups of 8 entries assembled in registers
+// by funnel shifts (after 8 appends the oldest entry sits in the low half of
+// q0) and stored whole, one 16-byte store per group; groups beyond the
+// capacity are counted, not stored.
+struct ListCol {
+    int4* col;
+    int ngroups;
+    unsigned q0, q1, q2, q3;
+    int cnt;
+    __device__ __forceinline__ ListCol(int4* c, int ng) : col(c), ngroups(ng), q0(0u), q1(0u), q2(0u), q3(0u), cnt(0) {}
+    __device__ __forceinline__ void shift_in(unsigned t) {
+        q0 = __builtin_amdgcn_alignbit(q1, q0, 16);
+        q1 = __builtin_amdgcn_alignbit(q2, q1, 16);
+        q2 = __builtin_amdgcn_alignbit(q3, q2, 16);
+        q3 = __builtin_amdgcn_alignbit(t, q3, 16);
+    }
+    __device__ __forceinline__ void push(unsigned t) {
+        shift_in(t);
+        ++cnt;
+        if ((cnt & 7) == 0) {
+            const int grp = (cnt >> 3) - 1;
+            if (grp < ngroups) col[(size_t)grp * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+        }
+    }
+    // the last group's free slots hold the window's sentinel offset
+    __device__ __forceinline__ void finish(unsigned pad) {
+        if ((cnt & 7) != 0 && (cnt >> 3) < ngroups) {
+            for (int p = cnt & 7; p < 8; ++p) shift_in(pad);
+            col[(size_t)(cnt >> 3) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+        }
+    }
+};
+
 // Verlet list of local atom i (work slot w, cell c of the tile's grid,
 // cell-ordered index k_i) from the staged window: 9 stencil rows of 3 cells
 // in the reference's order, `rsq <= cut2` in FP64 without contraction.  Row
@@ -235,35 +268,36 @@ struct NbArgs {
 // accepted); i itself is removed from the mask by its window offset, so full
 // lists read no atom index (half lists read the staged index for the j < i /
 // ghost filter).  The set bits are then appended in ascending order, which is
-// the reference's order, funnel-shifted 8 to a 16-byte group in registers,
-// and each full group is written with one store (one list column per lane).  Only
-// accepted candidates pay for the packing.  The last group's free slots hold
-// the window's sentinel offset.  Returns the count; groups beyond the
-// capacity are counted, not stored.  STAGED is a template parameter so that
-// the staged loop addresses LDS directly (ds_read) rather than through
-// generic pointers.
+// the reference's order (ListCol).  Only accepted candidates pay for the
+// packing.  Returns the count.
+//
+// HALF also writes the complementary column (nb_list2, count in cnt2): the
+// local neighbours j > i that the half list leaves to j's own list.  The
+// pair force then needs no reaction scatter: atom i gathers its half-list
+// entries and its complementary entries, which are exactly the pairs whose
+// reactions the half list would have sent to i.  STAGED is a template
+// parameter so that the staged loop addresses LDS directly (ds_read) rather
+// than through generic pointers.
 template <bool HALF, bool STAGED>
 __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const double4 pi, const int* cst, const NbArgs& a,
                                           const TileWin& W, const WinStage& sp, const double4* __restrict__ spos,
-                                          const int* __restrict__ parts, nb_t* __restrict__ nb_list) {
+                                          const int* __restrict__ parts, nb_t* __restrict__ nb_list,
+                                          nb_t* __restrict__ nb_list2, int& cnt2) {
     const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);
     const int wy = a.g.ty + 2;
     const int xo = cx - W.cx0;  // (cx - 1) - (cx0 - 1)
     const int per = a.g.tx + 3;
     const int s_own = (cz - (W.cz0 - 1)) * wy + (cy - (W.cy0 - 1));
     const int t_i = W.seg_off[s_own] + (k_i - W.seg_beg[s_own]);
-    int4* const col = reinterpret_cast<int4*>(nb_list) + (size_t)(w >> 6) * (size_t)a.cap * 8 + (w & 63);
-    const int ngroups = a.cap >> 3;
-    // the group being filled: entries are shifted in at the top (funnel
-    // shifts), so after 8 appends the oldest sits in the low half of q0
-    unsigned q0 = 0u, q1 = 0u, q2 = 0u, q3 = 0u;
-    int cnt = 0;
+    const size_t col_off = (size_t)(w >> 6) * (size_t)a.cap * 8 + (w & 63);
+    ListCol A(reinterpret_cast<int4*>(nb_list) + col_off, a.cap >> 3);
+    ListCol B(HALF ? reinterpret_cast<int4*>(nb_list2) + col_off : nullptr, a.cap >> 3);
     for (int r = 0; r < 9; ++r) {
         const int s = s_own + (r / 3 - 1) * wy + (r % 3 - 1);
         const int t0 = cst[s * per + xo], t1 = cst[s * per + xo + 3];
         for (int tc = t0; tc < t1; tc += 32) {
             const int te = min(tc + 32, t1);
-            unsigned m = 0u;
+            unsigned m = 0u, m2 = 0u;
             for (int tb = tc; tb < te; tb += 4) {
                 const unsigned b0 = 1u << (tb - tc);  // bit of candidate tb
 #pragma unroll
@@ -286,40 +320,35 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
                     const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                     const double rsq = dx * dx + dy * dy + dz * dz;
                     bool ok = rsq <= a.cut2;
-                    if (HALF) ok = ok && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                    if (HALF) {
+                        m2 |= (ok && j < a.n && j > i) ? b0 << u : 0u;
+                        ok = ok && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                    }
                     m |= ok ? b0 << u : 0u;
                 }
             }
-            m &= te - tc == 32 ? 0xffffffffu : (1u << (te - tc)) - 1u;  // candidates of this chunk only
+            const unsigned chunk = te - tc == 32 ? 0xffffffffu : (1u << (te - tc)) - 1u;  // candidates of this chunk only
+            m &= chunk;
             const unsigned d = (unsigned)(t_i - tc);
             m &= ~(d < 32u ? 1u << d : 0u);  // i is not its own neighbour
             while (m != 0u) {
-                const unsigned t = (unsigned)(tc + __builtin_ctz(m));
+                A.push((unsigned)(tc + __builtin_ctz(m)));
                 m &= m - 1u;
-                q0 = __builtin_amdgcn_alignbit(q1, q0, 16);
-                q1 = __builtin_amdgcn_alignbit(q2, q1, 16);
-                q2 = __builtin_amdgcn_alignbit(q3, q2, 16);
-                q3 = __builtin_amdgcn_alignbit(t, q3, 16);
-                ++cnt;
-                if ((cnt & 7) == 0) {
-                    const int grp = (cnt >> 3) - 1;
-                    if (grp < ngroups) col[(size_t)grp * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+            }
+            if (HALF) {
+                m2 &= chunk;  // j > i excludes i itself
+                while (m2 != 0u) {
+                    B.push((unsigned)(tc + __builtin_ctz(m2)));
+                    m2 &= m2 - 1u;
                 }
             }
         }
     }
-    if ((cnt & 7) != 0 && (cnt >> 3) < ngroups) {
-        // the last group's free slots hold the window's sentinel offset
-        const unsigned pad = (unsigned)win_size(W);
-        for (int p = cnt & 7; p < 8; ++p) {
-            q0 = __builtin_amdgcn_alignbit(q1, q0, 16);
-            q1 = __builtin_amdgcn_alignbit(q2, q1, 16);
-            q2 = __builtin_amdgcn_alignbit(q3, q2, 16);
-            q3 = __builtin_amdgcn_alignbit(pad, q3, 16);
-        }
-        col[(size_t)(cnt >> 3) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
-    }
-    return cnt;
+    const unsigned pad = (unsigned)win_size(W);
+    A.finish(pad);
+    if (HALF) B.finish(pad);
+    cnt2 = B.cnt;
+    return A.cnt;
 }
 
 }  // namespace tmd

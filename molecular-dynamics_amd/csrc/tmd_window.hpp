@@ -32,7 +32,6 @@ constexpr int kWinMax = 65535;     // window offsets are 16-bit
 // block's static LDS (window rows, reductions)
 constexpr int kLdsBudget = 80 * 1024 - 1024;
 constexpr int kStageBytes = 28;       // LDS bytes per staged window atom: x, y, z 24, index 4
-constexpr int kLdsHalfBudget = 144 * 1024;  // half-list LJ: window records + reaction accumulators (one block per CU)
 
 typedef unsigned short nb_t;
 

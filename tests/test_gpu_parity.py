@@ -296,3 +296,24 @@ def test_trajectory_bitwise_tile_shapes(tm, tile, threads):
     xo, vo, fo = orc.state()
     assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
     sim.close()
+
+
+@pytest.mark.parametrize("precision,tol_x", [("fp64", 1e-9), ("fp32mixed", 1e-6)])
+def test_half_list_trajectory_16cube(tm, precision, tol_x):
+    """Half lists at 16^3 (staged windows; the complementary columns carry
+    the reactions): a trajectory across a rebuild stays within rounding of
+    the oracle's half-list run."""
+    announce(f"half-list trajectory 16^3, 25 steps, {precision}")
+    aabb, m, x, v = perturbed_fcc(16, amp=0.05, seed=23)
+    sim, orc = make_pair(aabb, m, x, v, threads=8, precision=precision)
+    setup_both(sim, orc, True)
+    sim.lennard_jones(True, RC)
+    orc.lj(True, RC)
+    sim.run_lj(25, reneigh=20, dt=0.005, half=True)
+    orc.run_lj(25, reneigh=20, dt=0.005, half=True)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.max(np.abs(xg - xo)) < tol_x and np.max(np.abs(vg - vo)) < 100 * tol_x
+    rms, mx = force_error(fg, fo)
+    assert rms < (1e-12 if precision == "fp64" else 1e-5), (rms, mx)
+    sim.close()
