@@ -77,16 +77,25 @@ def test_neighbor_lists_exact(tm, half, half_mode):
         assert all(np.all(lo[i, :co[i]] < n) for i in range(n))
 
 
-def test_list_capacity_retry(tm):
-    announce("list capacity 1 -> grow and retry")
+@pytest.mark.parametrize("half", [False, True])
+def test_list_capacity_retry(tm, half):
+    """Capacity 1 grows and retries; with half lists the complementary
+    columns share the capacity, and the forces come out right."""
+    announce(f"list capacity 1 -> grow and retry, half={half}")
     aabb, m, x, v = perturbed_fcc(6, amp=0.1, seed=3)
     sim, orc = make_pair(aabb, m, x, v, nb_cap=1)
-    setup_both(sim, orc)
+    setup_both(sim, orc, half)
     cg, lg = sim.neighbor_lists()
     co, lo = orc.neighbor_lists()
     assert np.array_equal(cg, co)
     for i in range(len(cg)):
         assert np.array_equal(lg[i, :cg[i]], lo[i, :co[i]])
+    sim.lennard_jones(half, RC)
+    orc.lj(half, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    rms, mx = force_error(fg, fo)
+    assert rms < 1e-12 and mx < 1e-11, (rms, mx)
 
 
 def test_lj_full_fp64_bitwise(tm):
