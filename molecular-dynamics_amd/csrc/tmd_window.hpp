@@ -145,13 +145,23 @@ __device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const dou
     const int S = win_size(W);
     const bool staged = S <= wcap;
     if (staged) {
-        for (int t = threadIdx.x; t < S; t += blockDim.x) {
-            const int k = win_index(t, W);
-            const double4 p = spos[k];
-            sx[t] = p.x;
-            sy[t] = p.y;
-            sz[t] = p.z;
-            if (sj) sj[t] = parts[k];
+        // one thread group per window row, as in stage_window
+        const int nseg = W.nseg;
+        const int gsz = max(1, (int)blockDim.x / nseg);
+        const int ngrp = (int)blockDim.x / gsz;
+        const int g = (int)threadIdx.x / gsz, l = (int)threadIdx.x - g * gsz;
+        if (g < ngrp) {
+            for (int s = g; s < nseg; s += ngrp) {
+                const int off = W.seg_off[s], beg = W.seg_beg[s], len = W.seg_off[s + 1] - off;
+                for (int q = l; q < len; q += gsz) {
+                    const int t = off + q, k = beg + q;
+                    const double4 p = spos[k];
+                    sx[t] = p.x;
+                    sy[t] = p.y;
+                    sz[t] = p.z;
+                    if (sj) sj[t] = parts[k];
+                }
+            }
         }
     }
     __syncthreads();
@@ -191,13 +201,25 @@ __device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const d
     const int S = win_size(W);
     const bool staged = S + pad <= wcap;
     if (staged) {
-        for (int t = threadIdx.x; t < S; t += blockDim.x) {
-            const int k = win_index(t, W);
-            const double4 p = spos[k];
-            st.xyz[3 * t + 0] = p.x;
-            st.xyz[3 * t + 1] = p.y;
-            st.xyz[3 * t + 2] = p.z;
-            if (parts) st.j[t] = parts[k];
+        // the block is split into one thread group per window row (rows in
+        // turn when there are more rows than groups); a group copies its
+        // row's contiguous CSR range, so no offset-to-row search is needed
+        const int nseg = W.nseg;
+        const int gsz = max(1, (int)blockDim.x / nseg);
+        const int ngrp = (int)blockDim.x / gsz;
+        const int g = (int)threadIdx.x / gsz, l = (int)threadIdx.x - g * gsz;
+        if (g < ngrp) {
+            for (int s = g; s < nseg; s += ngrp) {
+                const int off = W.seg_off[s], beg = W.seg_beg[s], len = W.seg_off[s + 1] - off;
+                for (int q = l; q < len; q += gsz) {
+                    const int t = off + q, k = beg + q;
+                    const double4 p = spos[k];
+                    st.xyz[3 * t + 0] = p.x;
+                    st.xyz[3 * t + 1] = p.y;
+                    st.xyz[3 * t + 2] = p.z;
+                    if (parts) st.j[t] = parts[k];
+                }
+            }
         }
         if ((int)threadIdx.x < pad) {
             const int t = S + (int)threadIdx.x;
