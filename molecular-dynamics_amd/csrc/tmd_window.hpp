@@ -82,9 +82,17 @@ __device__ __forceinline__ void tile_window(int tile, const int* __restrict__ st
         W.nseg = nseg;
     }
     __syncthreads();
-    if (t == 0) {
-        W.seg_off[0] = 0;
-        for (int s = 0; s < nseg; ++s) W.seg_off[s + 1] += W.seg_off[s];
+    if (t < kWave) {
+        // the first wavefront turns the row lengths into offsets (inclusive
+        // scan, 64 rows at a time; blocks have at least one wavefront)
+        int carry = 0;
+        for (int s0 = 0; s0 < nseg; s0 += kWave) {
+            const int s = s0 + t;
+            const int inc = wave_inclusive_scan(s < nseg ? W.seg_off[s + 1] : 0);
+            if (s < nseg) W.seg_off[s + 1] = carry + inc;
+            carry += __shfl(inc, kWave - 1, kWave);
+        }
+        if (t == 0) W.seg_off[0] = 0;
     }
     __syncthreads();
 }
