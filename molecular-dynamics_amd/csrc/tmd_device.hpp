@@ -152,12 +152,4 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
     return x * q + min(x, r) + k;
 }
 
-// Neighbour-list layout: tiles of 64 work slots; inside a tile, entry k of
-// slot column l sits at (k / 4) * 256 + l * 4 + (k % 4), so one int4 load per
-// lane reads 4 consecutive entries and a wavefront reads 1 KiB contiguously.
-// The per-slot capacity is a multiple of 4.
-__device__ __forceinline__ size_t nb_slot(int w, int k, int cap) {
-    return (size_t)(w >> 6) * (size_t)cap * 64 + (size_t)(k >> 2) * 256 + (size_t)(w & 63) * 4 + (size_t)(k & 3);
-}
-
 }  // namespace tmd

@@ -110,18 +110,6 @@ __device__ __forceinline__ int win_index(int t, const TileWin& W) {
     return W.seg_beg[lo] + (t - W.seg_off[lo]);
 }
 
-// Window offsets [t0, t1) of the 3 cells x-1..x+1 at stencil row (oy, oz)
-// of an atom in cell (cx, cy, cz) of this tile.
-__device__ __forceinline__ void win_stencil_row(int cx, int cy, int cz, int oy, int oz, const int* __restrict__ start,
-                                                TileGeom g, const TileWin& W, int& t0, int& t1) {
-    const int y = cy + oy, z = cz + oz;
-    const int s = (z - (W.cz0 - 1)) * (g.ty + 2) + (y - (W.cy0 - 1));
-    const int row = (z * g.ny + y) * g.nx;
-    const int kb = start[row + cx - 1], ke = start[row + cx + 2];
-    t0 = W.seg_off[s] + (kb - W.seg_beg[s]);
-    t1 = t0 + (ke - kb);
-}
-
 // Window offsets of the cell starts of every window row, for the list
 // build: cst[s * (tx + 3) + (x - (cx0 - 1))] for x in [cx0 - 1, xhi + 1].
 // A local atom's stencil row then needs no global load.  Whole block.
