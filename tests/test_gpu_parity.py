@@ -196,6 +196,29 @@ def test_ref_lag_ghosts_trajectory(tm):
     assert np.array_equal(xg, xo) and np.array_equal(vg, vo)
 
 
+@pytest.mark.parametrize("shape", [(6, 6, 6), (4, 4, 4), (4, 7, 5)])
+def test_fused_loop_ghost_positions(tm, shape):
+    """Fused step loop (positions advanced in the mirror, ghosts refreshed
+    from their owners there): in the small boxes corner owners have the full
+    7 images.  The trajectory and the final ghost positions are the oracle's
+    bit for bit."""
+    announce(f"fused loop ghost positions {shape}, 25 steps")
+    aabb, m, x, v = perturbed_fcc(*shape, amp=0.05, seed=31)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    sim.run_lj(25, reneigh=20, dt=0.005)
+    orc.run_lj(25, reneigh=20, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    assert sim.nghost == orc.ng
+    _, gx, _, _ = sim.ghost_state()
+    assert np.array_equal(gx, orc.ghosts()["pos"])
+    sim.close()
+
+
 def test_half_list_trajectory_close(tm):
     announce("half-list trajectory vs oracle (tolerance)")
     aabb, m, x, v = perturbed_fcc(6, amp=0.05, seed=9)
