@@ -97,10 +97,11 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
     float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
     const int ng = (cnt + 7) >> 3;
     // the next group of 8 entries is loaded while the current one is used
-    int4 q = lp[0];  // group 0 of the column always exists (capacity >= 8)
+    // (streaming loads: each group is read once per pass)
+    int4 q = ld_stream(lp);  // group 0 of the column always exists (capacity >= 8)
     for (int gi = 0; gi < ng; ++gi) {
         const int4 e = q;
-        q = lp[(size_t)min(gi + 1, ng - 1) * 64];  // next group (clamped: always a valid load)
+        q = ld_stream(lp + (size_t)min(gi + 1, ng - 1) * 64);  // next group (clamped: always a valid load)
         // two passes of 4 entries (fewer live registers: 5 waves per SIMD)
 #pragma unroll 1
         for (int hb = 0; hb < 8; hb += 4) {
