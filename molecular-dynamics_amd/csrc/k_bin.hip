@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(kWave) k_tile_stats(const int* __restrict__ st
 __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__ start, const int* __restrict__ local_count,
                                                          const int* __restrict__ tile_start, TileGeom g,
                                                          const int* __restrict__ parts, int* __restrict__ work,
-                                                         int* __restrict__ work_k) {
+                                                         int* __restrict__ work_k, int* __restrict__ work_c) {
     __shared__ int coff[kMaxTile * kMaxTile * kMaxTile + 1];
     const int tile = blockIdx.x;
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
@@ -194,6 +194,7 @@ __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__
         for (int a = 0; a < coff[q + 1] - coff[q]; ++a) {
             work[w0 + coff[q] + a] = parts[b + a];
             work_k[w0 + coff[q] + a] = b + a;  // the atom's mirror index
+            work_c[w0 + coff[q] + a] = c;
         }
     }
 }
@@ -236,6 +237,7 @@ void bin_particles(Sim& s) {
     s.cell_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
     s.work.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
     s.work_k.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
+    s.work_c.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
     if (has_error()) return;
     BinArgs a{G.aabb[0], G.aabb[2], G.aabb[4], G.spacing, G.nx, G.ny, G.ncells, nall};
     mirror_invalidate(s);
@@ -276,7 +278,7 @@ void bin_particles(Sim& s) {
     exclusive_scan_int(s.work_start.as<int>(), s.tile_start.as<int>(), ntiles, s.tile_start.as<int>() + ntiles, s);
     hipLaunchKernelGGL(k_tile_fill_block, dim3(ntiles), dim3(256), 0, s.stream, (const int*)s.cell_start.as<int>(),
                        (const int*)s.cell_count.as<int>(), (const int*)s.tile_start.as<int>(), tg,
-                       (const int*)s.cell_parts.as<int>(), s.work.as<int>(), s.work_k.as<int>());
+                       (const int*)s.cell_parts.as<int>(), s.work.as<int>(), s.work_k.as<int>(), s.work_c.as<int>());
     TMD_HIP(hipGetLastError());
     s.windows_valid = 0;
     build_windows(s);

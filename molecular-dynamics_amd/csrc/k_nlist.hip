@@ -26,9 +26,11 @@ __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict_
     if (svel && j < n) svel[k] = vel[j];
 }
 
+// Per-atom inputs come in work order (mirror index work_k, cell work_c), so
+// a wavefront's reads of them are contiguous.
 template <bool HALF>
-__global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __restrict__ spos, const double4* __restrict__ pos,
-                                                        const int* __restrict__ cell_of, const int* __restrict__ inv,
+__global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __restrict__ spos, const int* __restrict__ work_k,
+                                                        const int* __restrict__ work_c,
                                                         const int* __restrict__ start, const int* __restrict__ parts,
                                                         const int* __restrict__ work, const int* __restrict__ tile_start,
                                                         NbArgs a, int* __restrict__ nb_count, nb_t* __restrict__ nb_list,
@@ -47,10 +49,11 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __rest
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
+        const int k = work_k[w];
         int cnt2 = 0;
-        const int cnt = staged ? nlist_atom<HALF, true>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list,
+        const int cnt = staged ? nlist_atom<HALF, true>(i, w, work_c[w], k, spos[k], cst, a, W, sp, spos, parts, nb_list,
                                                         nb_list2, cnt2)
-                               : nlist_atom<HALF, false>(i, w, cell_of[i], inv[i], pos[i], cst, a, W, sp, spos, parts, nb_list,
+                               : nlist_atom<HALF, false>(i, w, work_c[w], k, spos[k], cst, a, W, sp, spos, parts, nb_list,
                                                          nb_list2, cnt2);
         nb_count[w] = cnt;
         if (HALF) nb_count2[w] = cnt2;
@@ -159,7 +162,7 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         if (s.n > 0) {
 #define TMD_NLIST(H)                                                                                                   \
     hipLaunchKernelGGL(k_nlist<H>, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(), \
-                       (const double4*)s.pos.as<double4>(), (const int*)s.cell_of.as<int>(), (const int*)s.cell_inv.as<int>(), \
+                       (const int*)s.work_k.as<int>(), (const int*)s.work_c.as<int>(),                                      \
                        (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), \
                        (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(), s.nb_list.as<nb_t>(), \
                        s.nb_count2.as<int>(), s.nb_list2.as<nb_t>(), counters + 2)

@@ -125,6 +125,7 @@ struct Sim {
     DevBuf spos2;                      // second mirror: the force pass that advances positions writes the
                                        // new local positions here, then the two are swapped
     DevBuf work_k;                     // mirror (CSR) index of each work slot's atom
+    DevBuf work_c;                     // cell of each work slot's atom
     DevBuf svel;                       // local velocities in mirror order during the fused step loop
     DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
     DevBuf nb_count, nb_list;
