@@ -298,9 +298,12 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
         const int t0 = cst[s * per + xo], t1 = cst[s * per + xo + 3];
         for (int tc = t0; tc < t1; tc += 32) {
             const int te = min(tc + 32, t1);
+            // each batch of 4 candidates is shifted in at the low end (first
+            // candidate highest), which needs only small constant bits
             unsigned m = 0u, m2 = 0u;
+            int nin = 0;  // candidates shifted in
             for (int tb = tc; tb < te; tb += 4) {
-                const unsigned b0 = 1u << (tb - tc);  // bit of candidate tb
+                unsigned bq = 0u, bq2 = 0u;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     // staged: up to three candidates past te are read (window
@@ -322,12 +325,18 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
                     const double rsq = dx * dx + dy * dy + dz * dz;
                     bool ok = rsq <= a.cut2;
                     if (HALF) {
-                        m2 |= (ok && j < a.n && j > i) ? b0 << u : 0u;
+                        bq2 |= (ok && j < a.n && j > i) ? 8u >> u : 0u;
                         ok = ok && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
                     }
-                    m |= ok ? b0 << u : 0u;
+                    bq |= ok ? 8u >> u : 0u;
                 }
+                m = (m << 4) | bq;
+                if (HALF) m2 = (m2 << 4) | bq2;
+                nin += 4;
             }
+            // reversed, bit b is candidate tc + b (nin >= 4: the chunk is not empty)
+            m = __builtin_bitreverse32(m) >> (32 - nin);
+            if (HALF) m2 = __builtin_bitreverse32(m2) >> (32 - nin);
             const unsigned chunk = te - tc == 32 ? 0xffffffffu : (1u << (te - tc)) - 1u;  // candidates of this chunk only
             m &= chunk;
             const unsigned d = (unsigned)(t_i - tc);
