@@ -141,32 +141,19 @@ __device__ __forceinline__ int wave_sort_cell(int b, int e, int lim, int* __rest
     return __shfl(nl, 0, kWave);
 }
 
-// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
-// with a private L2.  Remap so that XCD x processes one contiguous range of
-// logical blocks: atoms that are neighbours in cell order (and so share
-// neighbour positions) then share an L2.  A bijection on [0, nblocks).
-// Streaming (non-temporal) accesses, for data a launch touches once (list
-// groups, for one): they do not displace the lines that neighbouring tiles
-// re-read from L2 (the staged window positions).
+// Streaming (non-temporal) load, for data a launch reads once (list groups):
+// it does not displace the lines that neighbouring tiles re-read from L2
+// (the staged window positions).
 typedef int tmd_v4i __attribute__((ext_vector_type(4)));
-typedef double tmd_v4d __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int4 ld_stream(const int4* p) {
     const tmd_v4i t = __builtin_nontemporal_load(reinterpret_cast<const tmd_v4i*>(p));
     return make_int4(t.x, t.y, t.z, t.w);
 }
-__device__ __forceinline__ void st_stream(int4* p, int4 v) {
-    const tmd_v4i t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, reinterpret_cast<tmd_v4i*>(p));
-}
-__device__ __forceinline__ double4 ld_stream(const double4* p) {
-    const tmd_v4d t = __builtin_nontemporal_load(reinterpret_cast<const tmd_v4d*>(p));
-    return make_double4(t.x, t.y, t.z, t.w);
-}
-__device__ __forceinline__ void st_stream(double4* p, double4 v) {
-    const tmd_v4d t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, reinterpret_cast<tmd_v4d*>(p));
-}
 
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with a private L2.  Remap so that XCD x processes one contiguous range of
+// logical blocks: atoms that are neighbours in cell order (and so share
+// neighbour positions) then share an L2.  A bijection on [0, nblocks).
 constexpr int kNumXcd = 8;
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {
     const int q = nblocks / kNumXcd, r = nblocks % kNumXcd;

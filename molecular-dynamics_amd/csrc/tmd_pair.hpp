@@ -301,8 +301,9 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
             // each batch of 4 candidates is shifted in at the low end (first
             // candidate highest), which needs only small constant bits
             unsigned m = 0u, m2 = 0u;
-            int nin = 0;  // candidates shifted in
-            for (int tb = tc; tb < te; tb += 4) {
+            const int nbat = (te - tc + 3) >> 2;  // batches (the loop test is then one compare)
+            for (int q = 0; q < nbat; ++q) {
+                const int tb = tc + 4 * q;
                 unsigned bq = 0u, bq2 = 0u;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -332,11 +333,10 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
                 }
                 m = (m << 4) | bq;
                 if (HALF) m2 = (m2 << 4) | bq2;
-                nin += 4;
             }
-            // reversed, bit b is candidate tc + b (nin >= 4: the chunk is not empty)
-            m = __builtin_bitreverse32(m) >> (32 - nin);
-            if (HALF) m2 = __builtin_bitreverse32(m2) >> (32 - nin);
+            // reversed, bit b is candidate tc + b (nbat >= 1: the chunk is not empty)
+            m = __builtin_bitreverse32(m) >> (32 - 4 * nbat);
+            if (HALF) m2 = __builtin_bitreverse32(m2) >> (32 - 4 * nbat);
             const unsigned chunk = te - tc == 32 ? 0xffffffffu : (1u << (te - tc)) - 1u;  // candidates of this chunk only
             m &= chunk;
             const unsigned d = (unsigned)(t_i - tc);
