@@ -148,9 +148,9 @@ __global__ void __launch_bounds__(kWave) k_tile_stats(const int* __restrict__ st
 }
 
 // Work order: one block per tile writes the tile's locals in (z, y, x)
-// cell order, one thread per cell (lanes of a wavefront are then atoms of
-// adjacent cells, whose neighbour windows overlap -- measured faster than
-// grouping lanes by list length).
+// cell order (lanes of a wavefront are then atoms of adjacent cells, whose
+// neighbour windows overlap -- measured faster than grouping lanes by list
+// length), with each atom's mirror index and cell.
 __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__ start, const int* __restrict__ local_count,
                                                          const int* __restrict__ tile_start, TileGeom g,
                                                          const int* __restrict__ parts, int* __restrict__ work,
@@ -188,14 +188,20 @@ __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__
         if (threadIdx.x == 0) coff[nc] = carry;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
-        const int c = ((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex);
-        const int b = start[c];
-        for (int a = 0; a < coff[q + 1] - coff[q]; ++a) {
-            work[w0 + coff[q] + a] = parts[b + a];
-            work_k[w0 + coff[q] + a] = b + a;  // the atom's mirror index
-            work_c[w0 + coff[q] + a] = c;
+    // one thread per local (coalesced writes): its cell is the last one whose
+    // offset does not exceed its place (empty cells are passed over)
+    for (int e = threadIdx.x; e < nloc; e += blockDim.x) {
+        int lo = 0, hi = nc - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (coff[mid] <= e) lo = mid;
+            else hi = mid - 1;
         }
+        const int c = ((cz0 + lo / (ex * ey)) * g.ny + (cy0 + (lo / ex) % ey)) * g.nx + (cx0 + lo % ex);
+        const int k = start[c] + (e - coff[lo]);
+        work[w0 + e] = parts[k];
+        work_k[w0 + e] = k;  // the atom's mirror index
+        work_c[w0 + e] = c;
     }
 }
 
