@@ -47,16 +47,25 @@ struct FaceArgs {
     PbcArgs pbc;
 };
 
+// mpos / mvel (the fused step loop's state, in mirror order): the local's
+// position and velocity are first copied back from there (k_unmirror's copy,
+// through the mirror index inv).
 __global__ void __launch_bounds__(kScanThreads) k_face_flags(double4* __restrict__ pos, FaceArgs a, unsigned char* __restrict__ face,
-                                                             int* __restrict__ totals) {
+                                                             int* __restrict__ totals, const int* __restrict__ inv,
+                                                             const double4* __restrict__ mpos,
+                                                             const double4* __restrict__ mvel, double4* __restrict__ vel) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int t[6] = {0, 0, 0, 0, 0, 0};
     if (i < a.n) {
-        double4 p = pos[i];
-        if (a.wrap) {
-            pbc_wrap(p, a.pbc);  // md_enforce_pbc, fused
-            pos[i] = p;
+        double4 p;
+        if (mpos || mvel) {
+            const int k = inv[i];
+            if (mpos) p = mpos[k];
+            if (mvel) vel[i] = mvel[k];
         }
+        if (!mpos) p = pos[i];
+        if (a.wrap) pbc_wrap(p, a.pbc);  // md_enforce_pbc, fused
+        if (a.wrap || mpos) pos[i] = p;
         const double c[3] = {p.x, p.y, p.z};
         int bits = 0;
 #pragma unroll
@@ -279,6 +288,13 @@ static void ensure_atom_capacity(Sim& s, int need);
 // scatter, in the reference's order.
 void build_ghosts(Sim& s, bool wrap) {
     if (has_error()) return;
+    // after the fused step loop the locals' positions and velocities live in
+    // the mirror: the selection pass copies them back (binning's inv is still
+    // that of the mirror)
+    const double4* mpos = s.pos_stale ? s.spos.as<double4>() : nullptr;
+    const double4* mvel = s.vel_mirrored ? s.svel.as<double4>() : nullptr;
+    s.pos_stale = 0;
+    s.vel_mirrored = 0;
     s.ng = 0;
     s.inv_valid = 0;
     mirror_invalidate(s);
@@ -298,7 +314,8 @@ void build_ghosts(Sim& s, bool wrap) {
     fa.pbc = pbc_args(s);
     TMD_HIP(hipMemsetAsync(counters + 8, 0, 6 * sizeof(int), s.stream));
     hipLaunchKernelGGL(k_face_flags, dim3(div_up(s.n, kScanThreads)), dim3(kScanThreads), 0, s.stream, s.pos.as<double4>(), fa,
-                       s.face.as<unsigned char>(), counters + 8);
+                       s.face.as<unsigned char>(), counters + 8, (const int*)s.cell_inv.as<int>(), mpos, mvel,
+                       s.vel.as<double4>());
     int tot[6] = {0, 0, 0, 0, 0, 0};
     TMD_HIP(hipMemcpyAsync(tot, counters + 8, sizeof tot, hipMemcpyDeviceToHost, s.stream));
     TMD_HIP(hipStreamSynchronize(s.stream));

@@ -783,8 +783,7 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         s.vel_mirror_wanted = 1;  // the rebuild's mirror pass moves them again
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {
-                sync_pos_from_mirror(s);  // the rebuild reads pos and vel
-                rebuild(s, hf, cutoff + skin);
+                rebuild(s, hf, cutoff + skin);  // its ghost selection pass restores pos and vel from the mirror
             } else {
                 hipEvent_t ev = nullptr;
                 s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
