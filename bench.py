@@ -98,6 +98,16 @@ def fp32_force_error(tm, sim):
     return float(np.sqrt(np.mean(np.sum(d * d, axis=1))) / rms), float(np.max(np.abs(d)) / rms)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sample_steps=20):
     """FP64 oracle (test infrastructure, the checker) on the host cores: the
     same 80^3 workload, `sample_steps` steps including one rebuild."""
@@ -112,7 +122,8 @@ def cpu_baseline(sample_steps=20):
     s.close()
     return {"value": n * sample_steps / (t1 - t0), "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle, FCC 80^3 ({n} atoms), {sample_steps} steps incl. 1 rebuild, "
-                      f"{threads} OpenMP threads (list build and pair force threaded), {t1 - t0:.1f} s"}
+                      f"{threads} OpenMP threads (list build and pair force threaded), {t1 - t0:.1f} s",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def c1_leg(tm, nc=32, steps=100, cpu=True):
@@ -146,6 +157,13 @@ def c1_leg(tm, nc=32, steps=100, cpu=True):
         out.update({"cpu_seconds": c1 - c0, "cpu_atom_steps_per_s": n * steps / (c1 - c0), "cpu_threads": threads,
                     "cpu_T": tc["T"], "cpu_PE": tc["PE"], "cpu_P": tc["P"],
                     "gpu_over_cpu": (c1 - c0) / (t1 - t0)})
+        # one thread, on a 20-step sample of the same run
+        s1 = po.lj_benchmark(nc, threads=1)
+        d0 = time.perf_counter()
+        s1.run_lj(20, REN, DT)
+        d1 = time.perf_counter()
+        s1.close()
+        out.update({"cpu_1thread_atom_steps_per_s": n * 20 / (d1 - d0), "cpu_1thread_sample_steps": 20})
     return out
 
 
@@ -275,6 +293,41 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
 
 
+def hbm_copy_gbs_child():
+    """hbm_copy_gbs in a child process of its own (torch sets up the device
+    there, apart from the library's use of it in this process)."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bw"], capture_output=True, text=True,
+                           timeout=300)
+        if r.returncode != 0:
+            print(f"# copy bandwidth leg failed: {r.stderr[-400:]}", file=sys.stderr, flush=True)
+            return None
+        return json.loads(r.stdout.strip().splitlines()[-1])["copy_GBps"]
+    except (subprocess.TimeoutExpired, ValueError, KeyError, IndexError) as e:
+        print(f"# copy bandwidth leg failed: {e}", file=sys.stderr, flush=True)
+        return None
+
+
+def hbm_copy_gbs(nbytes=1 << 30, reps=5):
+    """Achievable HBM bandwidth on this device: a plain 1 GiB device-to-device
+    copy (torch), timed with events; bytes read + written per second."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def load_traffic():
     """HBM bytes per launch of the force kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
@@ -299,9 +352,14 @@ def main():
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM granular-gas leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
+    ap.add_argument("--copy-bw", action="store_true", help="only measure the device copy bandwidth and exit")
+    ap.add_argument("--no-copy-bw", action="store_true", help="skip the device copy bandwidth leg")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.copy_bw:
+        print(json.dumps({"copy_GBps": hbm_copy_gbs()}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU decomposition is design-only "
@@ -356,6 +414,7 @@ def main():
     t_force = r32["force_ms_avg"] * 1e-3
     achieved = nbytes / t_force / 1e9
     traffic = load_traffic()
+    copy_gbs = None if args.no_copy_bw else hbm_copy_gbs_child()
     line = {
         "metric": METRIC,
         "value": value,
@@ -375,7 +434,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_lj_full<FP32-mixed, kick fused>", "bytes_per_launch": nbytes,
-                     "avg_launch_ms": r32["force_ms_avg"]},
+                     "avg_launch_ms": r32["force_ms_avg"],
+                     "copy_measured": copy_gbs, "frac_of_copy": achieved / copy_gbs if copy_gbs else None},
         "fp32_vs_fp64_force_err": {"rms_rel": err_rms, "max_abs_over_rms": err_max},
         "pair_interactions_per_s": value * pairs_per_atom,
         "list_entries_per_s": value * entries / n,

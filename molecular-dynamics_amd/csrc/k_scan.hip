@@ -21,6 +21,31 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* __restric
 
 // One block: exclusive scan of the tile sums in place; grand total to *total.
 __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __restrict__ total) {
+    // up to kPer items per thread: one block-wide scan of per-thread sums
+    // (a few microseconds less than 256 items at a time for thousands of tiles)
+    constexpr int kPer = 16;
+    if (ntiles <= kScanThreads * kPer) {
+        const int per = (ntiles + kScanThreads - 1) / kScanThreads;
+        const int base = (int)threadIdx.x * per;
+        int v[kPer];
+        int sum = 0;
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int idx = base + it;
+            v[it] = (it < per && idx < ntiles) ? tile_sums[idx] : 0;
+            sum += v[it];
+        }
+        int tot;
+        int run = block_exclusive_scan(sum, &tot);
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int idx = base + it;
+            if (it < per && idx < ntiles) tile_sums[idx] = run;
+            run += v[it];
+        }
+        if (threadIdx.x == 0 && total) *total = tot;
+        return;
+    }
     int carry = 0;
     for (int base = 0; base < ntiles; base += kScanThreads) {
         int idx = base + threadIdx.x;
