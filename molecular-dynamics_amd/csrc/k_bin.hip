@@ -266,6 +266,7 @@ void bin_particles(Sim& s) {
                            (const int*)s.cell_start.as<int>(), G.ncells, s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(),
                            s.cell_inv.as<int>());
     s.inv_valid = 1;
+    build_ghost_src(s);
     // tile work order
     choose_tiles(s);
     const TileGeom tg = s.tiles;

@@ -249,6 +249,50 @@ __global__ void k_ghost_step(double4* __restrict__ pos, const int* __restrict__ 
     if (k >= 0) spos[k] = q;
 }
 
+// Mirror indices of each ghost's owner and of the ghost itself, so the
+// step loop's ghost pass reads them in ghost order instead of through inv.
+__global__ void k_ghost_src(const int* __restrict__ g_owner, const int* __restrict__ inv, int n, int ng,
+                            int* __restrict__ g_ko, int* __restrict__ g_kg) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    g_ko[g] = inv[g_owner[g]];
+    g_kg[g] = inv[n + g];
+}
+
+void build_ghost_src(Sim& s) {
+    s.gsrc_valid = 0;
+    if (has_error() || s.ng == 0) return;
+    s.g_ko.reserve(sizeof(int) * (size_t)s.cap_all, false, s.stream);
+    s.g_kg.reserve(sizeof(int) * (size_t)s.cap_all, false, s.stream);
+    if (has_error()) return;
+    const int nt = 256;
+    hipLaunchKernelGGL(k_ghost_src, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, (const int*)s.g_owner.as<int>(),
+                       (const int*)s.cell_inv.as<int>(), s.n, s.ng, s.g_ko.as<int>(), s.g_kg.as<int>());
+    TMD_HIP(hipGetLastError());
+    s.gsrc_valid = 1;
+}
+
+// Ghost refresh inside the fused step loop, mirror only: owner (current in
+// the mirror) + image shift, k_ghost_step's arithmetic.  pos is left for
+// sync_pos_from_mirror.
+__global__ void k_ghost_step_mirror(const int* __restrict__ g_ko, const int* __restrict__ g_kg,
+                                    const int* __restrict__ g_code, int ng, double L0, double L1, double L2,
+                                    double4* __restrict__ spos) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int kg = g_kg[g];
+    if (kg < 0) return;
+    int ix, iy, iz;
+    img_decode(g_code[g] & 0xff, ix, iy, iz);
+    const double4 p = spos[g_ko[g]];
+    double4 q;
+    q.x = p.x + L0 * (double)ix;
+    q.y = p.y + L1 * (double)iy;
+    q.z = p.z + L2 * (double)iz;
+    q.w = p.w;
+    spos[kg] = q;
+}
+
 // REF_LAG refresh (diagnostic): the reference's pack from the immediate
 // source (which may be a not-yet-refreshed ghost) then unpack.
 __global__ void k_ghost_pack_lag(const double4* __restrict__ pos, const int* __restrict__ g_src, const int* __restrict__ g_code,
@@ -295,8 +339,10 @@ void build_ghosts(Sim& s, bool wrap) {
     const double4* mvel = s.vel_mirrored ? s.svel.as<double4>() : nullptr;
     s.pos_stale = 0;
     s.vel_mirrored = 0;
+    s.ghost_pos_stale = 0;  // the ghosts are made anew
     s.ng = 0;
     s.inv_valid = 0;
+    s.gsrc_valid = 0;
     mirror_invalidate(s);
     if (s.n == 0) return;
     const Geom& G = s.g;
@@ -394,7 +440,17 @@ __global__ void k_unmirror(const double4* __restrict__ spos, const int* __restri
 
 // Positions (and velocities) of the fused step loop back into atom order.
 void sync_pos_from_mirror(Sim& s) {
-    if (has_error() || (!s.pos_stale && !s.vel_mirrored)) return;
+    if (has_error()) return;
+    if (s.ghost_pos_stale) {
+        // the ghosts' positions in pos, from their owners in the mirror
+        if (s.ng > 0)
+            hipLaunchKernelGGL(k_ghost_step, dim3(div_up(s.ng, 256)), dim3(256), 0, s.stream, s.pos.as<double4>(),
+                               (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0],
+                               s.g.len[1], s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+        TMD_HIP(hipGetLastError());
+        s.ghost_pos_stale = 0;
+    }
+    if (!s.pos_stale && !s.vel_mirrored) return;
     const int nt = 256;
     if (s.nbinned > 0)
         hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.spos.as<double4>(),
@@ -440,7 +496,14 @@ void step_refresh(Sim& s) {
         return;
     }
     const int nt = 256;
-    if (s.ghost_stale) {
+    if (s.ghost_stale && s.pos_stale && s.gsrc_valid) {
+        // fused step loop: the ghosts in the mirror only (pos at the end)
+        if (s.ng > 0)
+            hipLaunchKernelGGL(k_ghost_step_mirror, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream,
+                               (const int*)s.g_ko.as<int>(), (const int*)s.g_kg.as<int>(), (const int*)s.g_code.as<int>(),
+                               s.ng, s.g.len[0], s.g.len[1], s.g.len[2], s.spos.as<double4>());
+        s.ghost_pos_stale = 1;
+    } else if (s.ghost_stale) {
         // locals are current in the mirror: rewrite the ghosts only
         if (s.ng > 0)
             hipLaunchKernelGGL(k_ghost_step, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
@@ -473,6 +536,7 @@ static void ensure_atom_capacity(Sim& s, int need) {
     s.cell_inv.reserve(sizeof(int) * (size_t)cap, false, s.stream);
     s.face.reserve((size_t)cap, true, s.stream);
     s.inv_valid = 0;
+    s.gsrc_valid = 0;
     s.cap_all = cap;
 }
 

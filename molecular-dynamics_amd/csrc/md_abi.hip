@@ -309,6 +309,8 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     s.pos_stale = 0;
     s.vel_mirrored = 0;
     s.vel_mirror_wanted = 0;
+    s.ghost_pos_stale = 0;
+    s.gsrc_valid = 0;
     s.h_mass.assign(masses, masses + nparticles);
     s.h_pos.resize(3 * (size_t)nparticles);
     s.h_vel.resize(3 * (size_t)nparticles);
@@ -439,7 +441,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.g_ko, &s.g_kg, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel,
                       &s.nb_count, &s.nb_list, &s.nb_count2, &s.nb_list2,
                       &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
@@ -454,6 +456,8 @@ void md_deallocate_grid() {
     s.pos_stale = 0;
     s.vel_mirrored = 0;
     s.vel_mirror_wanted = 0;
+    s.ghost_pos_stale = 0;
+    s.gsrc_valid = 0;
     s.nbinned = 0;
     s.dev_valid = false;
 }

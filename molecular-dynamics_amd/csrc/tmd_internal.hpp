@@ -99,6 +99,8 @@ struct Sim {
     int pos_stale = 0;      // local positions in pos lag the mirror (fused step loop; sync_pos_from_mirror)
     int vel_mirrored = 0;   // local velocities live in svel, in mirror order (fused step loop)
     int vel_mirror_wanted = 0;  // fused step loop: refresh_mirror also moves the velocities to svel
+    int gsrc_valid = 0;     // g_ko / g_kg match the ghost table and the binning
+    int ghost_pos_stale = 0;  // ghosts in pos lag the mirror (fused step loop; sync_pos_from_mirror)
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
@@ -128,6 +130,7 @@ struct Sim {
     DevBuf work_c;                     // cell of each work slot's atom
     DevBuf svel;                       // local velocities in mirror order during the fused step loop
     DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
+    DevBuf g_ko, g_kg;                 // per ghost: mirror index of its owner and its own (-1: outside the grid)
     DevBuf nb_count, nb_list;
     DevBuf nb_count2, nb_list2;        // half lists: the complementary columns (local j > i)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
@@ -172,6 +175,7 @@ void refresh_ghosts(Sim& s);
 // fresh ghost positions and the cell-ordered mirror of locals and ghosts in
 // one pass (after positions were advanced inside the force pass)
 void step_refresh(Sim& s);
+void build_ghost_src(Sim& s);  // g_ko / g_kg (binning, after the CSR is built)
 void sync_pos_from_mirror(Sim& s);
 void mirror_velocities(Sim& s, bool into_mirror);
 
