@@ -276,6 +276,44 @@ __global__ void k_b_ghost_step(const RepDesc* __restrict__ R, const int* __restr
     if (k >= 0) spos[k] = q;
 }
 
+// mirror indices of each ghost's owner and of the ghost itself (after the
+// binning), read in slot order by the step loop's ghost pass
+__global__ void k_b_ghost_src(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep, const int* __restrict__ dyn,
+                              const int* __restrict__ g_owner, const int* __restrict__ inv, int* __restrict__ g_ko,
+                              int* __restrict__ g_kg) {
+    const int r = gb_rep[blockIdx.x];
+    const RepDesc& D = R[r];
+    const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
+    if (g >= dyn[r * DY_STRIDE + DY_NG]) return;
+    const int s = D.a0 + D.n + g;
+    g_ko[s] = inv[D.a0 + g_owner[s]];
+    g_kg[s] = inv[s];
+}
+
+// the step loop's ghost pass, mirror only (k_b_ghost_step's arithmetic); the
+// ghosts' pos entries are written by the full pass at the end of a run
+__global__ void k_b_ghost_step_mirror(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep,
+                                      const int* __restrict__ dyn, const int* __restrict__ g_code,
+                                      const int* __restrict__ g_ko, const int* __restrict__ g_kg,
+                                      double4* __restrict__ spos) {
+    const int r = gb_rep[blockIdx.x];
+    const RepDesc& D = R[r];
+    const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
+    if (g >= dyn[r * DY_STRIDE + DY_NG]) return;
+    const int s = D.a0 + D.n + g;
+    const int k = g_kg[s];
+    if (k < 0) return;
+    int ix, iy, iz;
+    b_img_decode(g_code[s], ix, iy, iz);
+    const double4 p = spos[g_ko[s]];
+    double4 q;
+    q.x = p.x + D.len[0] * (double)ix;
+    q.y = p.y + D.len[1] * (double)iy;
+    q.z = p.z + D.len[2] * (double)iz;
+    q.w = p.w;
+    spos[k] = q;
+}
+
 // pos of every local from its mirror entry (before a rebuild and at the end
 // of a run)
 __global__ void k_b_unmirror(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, const int* __restrict__ inv,
@@ -400,13 +438,19 @@ __global__ void __launch_bounds__(64) k_b_tile_fill(const RepDesc* __restrict__ 
         coff[nc] = o;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
-        const int c = ((cz0 + q / (ex * ey)) * g.ny + (cy0 + (q / ex) % ey)) * g.nx + (cx0 + q % ex);
-        const int b = st[c];
-        for (int a = 0; a < coff[q + 1] - coff[q]; ++a) {
-            work[w0 + coff[q] + a] = parts[b + a];
-            work_k[w0 + coff[q] + a] = b + a;
+    // one thread per local (coalesced writes): its cell is the last one whose
+    // offset does not exceed its place (empty cells are passed over)
+    for (int e = threadIdx.x; e < nloc; e += blockDim.x) {
+        int lo = 0, hi = nc - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (coff[mid] <= e) lo = mid;
+            else hi = mid - 1;
         }
+        const int c = ((cz0 + lo / (ex * ey)) * g.ny + (cy0 + (lo / ex) % ey)) * g.nx + (cx0 + lo % ex);
+        const int k = st[c] + (e - coff[lo]);
+        work[w0 + e] = parts[k];
+        work_k[w0 + e] = k;
     }
 }
 
@@ -610,7 +654,7 @@ struct Batch {
     int threads = kTileThreads;      // tile block size in use (cfg.threads, or automatic at set-up)
     DevBuf dR, sb_rep, eb_rep, cb_rep, gb_rep, tile_rep, cell_rep;
     DevBuf pos, vel, force, g_owner, g_code, cell_of, cell_slot, cell_inv, parts, spos, spos2;
-    DevBuf cell_count, cell_start, tile_count, tile_start, work, work_k;
+    DevBuf cell_count, cell_start, tile_count, tile_start, work, work_k, g_ko, g_kg;
     DevBuf sb_cnt0, sb_cnt1, scan_tmp, nb_count, nb_list;
     DevBuf dyn, flags, epart, kpart, tout;
     DevBuf snap_pos, snap_vel, snap_force;
@@ -627,7 +671,7 @@ struct Batch {
         drop_graphs();
         DevBuf* bufs[] = {&dR, &sb_rep, &eb_rep, &cb_rep, &gb_rep, &tile_rep, &cell_rep, &pos, &vel, &force, &g_owner,
                           &g_code, &cell_of, &cell_slot, &cell_inv, &parts, &spos, &spos2, &cell_count, &cell_start,
-                          &tile_count, &tile_start, &work, &work_k, &sb_cnt0, &sb_cnt1, &scan_tmp, &nb_count, &nb_list, &dyn, &flags, &epart,
+                          &tile_count, &tile_start, &work, &work_k, &g_ko, &g_kg, &sb_cnt0, &sb_cnt1, &scan_tmp, &nb_count, &nb_list, &dyn, &flags, &epart,
                           &kpart, &tout, &snap_pos, &snap_vel, &snap_force};
         if (stream) (void)hipStreamSynchronize(stream);
         for (DevBuf* b : bufs) b->release();
@@ -742,6 +786,8 @@ static void layout(Batch& B) {
     B.snap_force.reserve(sizeof(double4) * S, false, st);
     B.g_owner.reserve(sizeof(int) * S, false, st);
     B.g_code.reserve(sizeof(int) * S, false, st);
+    B.g_ko.reserve(sizeof(int) * S, false, st);
+    B.g_kg.reserve(sizeof(int) * S, false, st);
     B.cell_of.reserve(sizeof(int) * S, false, st);
     B.cell_slot.reserve(sizeof(int) * S, false, st);
     B.cell_inv.reserve(sizeof(int) * S, false, st);
@@ -825,6 +871,9 @@ static void enq_rebuild(Batch& B) {
     hipLaunchKernelGGL(k_b_bin_sort, dim3(div_up(B.n_cells, kBlk / kWave)), dim3(kBlk), 0, st, R, (const int*)B.cell_rep.as<int>(),
                        (const int*)B.cell_start.as<int>(), B.n_cells, B.parts.as<int>(), B.cell_count.as<int>(),
                        B.cell_inv.as<int>());
+    hipLaunchKernelGGL(k_b_ghost_src, dim3(B.n_gb), dim3(kBlk), 0, st, R, (const int*)B.gb_rep.as<int>(),
+                       (const int*)B.dyn.as<int>(), (const int*)B.g_owner.as<int>(), (const int*)B.cell_inv.as<int>(),
+                       B.g_ko.as<int>(), B.g_kg.as<int>());
     hipLaunchKernelGGL(k_b_tile_stats, dim3(div_up(B.n_tiles, kBlk)), dim3(kBlk), 0, st, R, (const int*)B.tile_rep.as<int>(),
                        (const int*)B.cell_start.as<int>(), (const int*)B.cell_count.as<int>(), B.n_tiles,
                        B.tile_count.as<int>(), B.flags.as<int>());
@@ -870,6 +919,12 @@ static void enq_nlist(Batch& B, const DevBuf& m) {
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), a, B.nb_count.as<int>(),
                        B.nb_list.as<nb_t>(), B.flags.as<int>());
     TMD_HIP(hipGetLastError());
+}
+
+static void enq_ghost_step_mirror(Batch& B, DevBuf& m) {
+    hipLaunchKernelGGL(k_b_ghost_step_mirror, dim3(B.n_gb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
+                       (const int*)B.gb_rep.as<int>(), (const int*)B.dyn.as<int>(), (const int*)B.g_code.as<int>(),
+                       (const int*)B.g_ko.as<int>(), (const int*)B.g_kg.as<int>(), m.as<double4>());
 }
 
 static void enq_ghost_step(Batch& B, DevBuf& m) {
@@ -933,13 +988,16 @@ static void enq_steps(Batch& B, int steps) {
             enq_mirror(B, *cur);
             enq_nlist(B, *cur);
         } else {
-            enq_ghost_step(B, *cur);
+            enq_ghost_step_mirror(B, *cur);
         }
         const bool adv = j + 1 < steps;
         enq_force(B, true, adv, *cur, *nxt);
         if (adv) std::swap(cur, nxt);
     }
-    if (steps > 0) enq_unmirror(B, *cur);
+    if (steps > 0) {
+        enq_ghost_step(B, *cur);  // the ghosts' pos entries (and the same mirror entries)
+        enq_unmirror(B, *cur);
+    }
     TMD_HIP(hipGetLastError());
 }
 
@@ -1045,6 +1103,7 @@ static std::vector<unsigned long long> graph_signature(const Batch& B) {
     const DevBuf* bufs[] = {&B.dR, &B.sb_rep, &B.eb_rep, &B.cb_rep, &B.gb_rep, &B.tile_rep, &B.cell_rep, &B.pos, &B.vel,
                             &B.force, &B.g_owner, &B.g_code, &B.cell_of, &B.cell_slot, &B.cell_inv, &B.parts, &B.spos,
                             &B.spos2, &B.cell_count, &B.cell_start, &B.tile_count, &B.tile_start, &B.work, &B.work_k,
+                            &B.g_ko, &B.g_kg,
                             &B.sb_cnt0, &B.sb_cnt1,
                             &B.scan_tmp, &B.nb_count, &B.nb_list, &B.dyn, &B.flags};
     for (const DevBuf* b : bufs) s.push_back((unsigned long long)(uintptr_t)b->p);
