@@ -474,8 +474,8 @@ __global__ void k_b_mirror(const double4* __restrict__ pos, const int* __restric
 // ---------------------------------------------------------------------------
 // neighbour lists and forces: one block per tile of any replica
 __global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
-                                                          const double4* __restrict__ spos, const double4* __restrict__ pos,
-                                                          const int* __restrict__ cell_of, const int* __restrict__ inv,
+                                                          const double4* __restrict__ spos, const int* __restrict__ work_k,
+                                                          const int* __restrict__ cell_of,
                                                           const int* __restrict__ start,
                                                           const int* __restrict__ parts, const int* __restrict__ work,
                                                           const int* __restrict__ tile_start, NbArgs a,
@@ -499,10 +499,11 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __re
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int i = work[w];
+        const int k = work_k[w];  // its mirror index (the mirror is current)
         int unused = 0;
-        const int cnt = staged ? nlist_atom<false, true>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list,
+        const int cnt = staged ? nlist_atom<false, true>(i, w, cell_of[i], k, spos[k], cst, aa, W, sp, spos, parts, nb_list,
                                                          nullptr, unused)
-                               : nlist_atom<false, false>(i, w, cell_of[i], inv[i], pos[i], cst, aa, W, sp, spos, parts, nb_list,
+                               : nlist_atom<false, false>(i, w, cell_of[i], k, spos[k], cst, aa, W, sp, spos, parts, nb_list,
                                                           nullptr, unused);
         nb_count[w] = cnt;
         if (cnt > a.cap) atomicMax(&flags[FL_LIST], cnt);
@@ -913,8 +914,8 @@ static void enq_nlist(Batch& B, const DevBuf& m) {
     a.cut2 = cut * cut;
     const size_t lds = (size_t)a.wcap * kStageBytes + rsb;
     hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.tile_rep.as<int>(), (const double4*)m.as<double4>(), (const double4*)B.pos.as<double4>(),
-                       (const int*)B.cell_of.as<int>(), (const int*)B.cell_inv.as<int>(), (const int*)B.cell_start.as<int>(),
+                       (const int*)B.tile_rep.as<int>(), (const double4*)m.as<double4>(), (const int*)B.work_k.as<int>(),
+                       (const int*)B.cell_of.as<int>(), (const int*)B.cell_start.as<int>(),
                        (const int*)B.parts.as<int>(),
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), a, B.nb_count.as<int>(),
                        B.nb_list.as<nb_t>(), B.flags.as<int>());
