@@ -256,8 +256,15 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS,
             const double4 vi = vel[i];
             double fx = 0.0, fy = 0.0, fz = 0.0;
             auto entries = [&](const nb_t* __restrict__ L, int cnt) {
+                // groups of 8 entries, one 16-byte streaming load each
+                const int4* lp = reinterpret_cast<const int4*>(L + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+                int4 e = make_int4(0, 0, 0, 0);
                 for (int k = 0; k < cnt; ++k) {
-                    const int t = L[nb16_slot(w, k, a.cap)];
+                    if ((k & 7) == 0) e = ld_stream(lp + (size_t)(k >> 3) * 64);
+                    // entry k & 7 of the group: shift the group's words down
+                    // as they are used (no indexed register access)
+                    const int t = (k & 1) ? (int)((unsigned)e.x >> 16) : (e.x & 0xffff);
+                    if (k & 1) e = make_int4(e.y, e.z, e.w, 0);
                     int j;
                     double xj, yj, zj;
                     if (STAGED) {
