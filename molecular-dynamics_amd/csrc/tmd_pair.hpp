@@ -102,8 +102,10 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
     for (int gi = 0; gi < ng; ++gi) {
         const int4 e = q;
         q = ld_stream(lp + (size_t)min(gi + 1, ng - 1) * 64);  // next group (clamped: always a valid load)
-        // two passes of 4 entries (fewer live registers: 5 waves per SIMD)
-#pragma unroll 1
+        // two passes of 4 entries (fewer live registers than 8 at once),
+        // unrolled: the compiler then overlaps one pass's LDS reads with the
+        // other's arithmetic
+#pragma unroll
         for (int hb = 0; hb < 8; hb += 4) {
         const int k0 = gi * 8 + hb;
         if (k0 >= cnt) break;  // this lane's entries are done
