@@ -75,18 +75,9 @@ int tile_block_threads(int requested, int max_locals) {
 
 int window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kStageBytes); }  // + sentinel slot
 
-void build_windows(Sim& s) {
-    if (has_error()) return;
-    int* counters = s.counters.as<int>();
-    const Geom& G = s.g;
-    const int ntiles = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
-    // counters[6] = largest tile window, counters[7] = most locals in a tile (from k_tile_stats)
-    int host[4] = {0, 0, 0, 0};
-    TMD_HIP(hipMemcpyAsync(&host[0], s.tile_start.as<int>() + ntiles, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    TMD_HIP(hipMemcpyAsync(&host[1], s.cell_start.as<int>() + G.ncells, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    TMD_HIP(hipMemcpyAsync(&host[2], counters + 6, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    TMD_HIP(hipStreamSynchronize(s.stream));
-    if (has_error()) return;
+// The window statistics (locals placed, binned atoms, largest window, most
+// locals in a tile) decide the LDS size and block size of the tile kernels.
+static void apply_window_stats(Sim& s, const int* host) {
     if (host[0] != s.n) {
         char msg[200];
         snprintf(msg, sizeof msg,
@@ -116,16 +107,64 @@ void build_windows(Sim& s) {
     s.windows_valid = 1;
 }
 
+// A rebuild inside the fused step loop (s.defer_window_sync, with statistics
+// from an earlier binning at hand) only queues the copy of the statistics
+// into pinned host memory: its list build runs with the previous window and
+// block sizes (a larger window is then read from the mirror, a fuller tile
+// takes a second pass of its block) and reads the new statistics after its
+// own synchronisation (finish_windows), saving one host round trip.
+void build_windows(Sim& s) {
+    if (has_error()) return;
+    int* counters = s.counters.as<int>();
+    const Geom& G = s.g;
+    const int ntiles = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    const bool defer = s.defer_window_sync && s.max_window > 0;
+    int local[4] = {0, 0, 0, 0};
+    int* host = local;
+    if (defer) {
+        if (!s.h_winstats) TMD_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_winstats), 4 * sizeof(int)));
+        if (has_error()) return;
+        host = s.h_winstats;
+    }
+    // counters[6] = largest tile window, counters[7] = most locals in a tile (from k_tile_stats)
+    TMD_HIP(hipMemcpyAsync(&host[0], s.tile_start.as<int>() + ntiles, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipMemcpyAsync(&host[1], s.cell_start.as<int>() + G.ncells, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipMemcpyAsync(&host[2], counters + 6, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    if (defer) {
+        const int bound = std::max(s.n + s.ng, 1);  // binned atoms, at most
+        s.spos.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+        s.spos2.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+        if (has_error()) return;
+        s.windows_pending = 1;
+        s.windows_valid = 1;
+        return;
+    }
+    TMD_HIP(hipStreamSynchronize(s.stream));
+    if (has_error()) return;
+    apply_window_stats(s, host);
+}
+
+void finish_windows(Sim& s) {
+    if (!s.windows_pending) return;
+    s.windows_pending = 0;
+    TMD_HIP(hipStreamSynchronize(s.stream));  // the copies were queued before
+    if (has_error()) return;
+    apply_window_stats(s, s.h_winstats);
+}
+
 void refresh_mirror(Sim& s) {
-    if (has_error() || s.nbinned == 0) return;
+    // the grid covers every atom that can be binned; the kernel stops at the
+    // binned count on the device (the host's may still be pending)
+    const int bound = s.n + s.ng;
+    if (has_error() || bound == 0) return;
     const int nt = 256;
     // inside the fused step loop the velocities follow the new order too
     const bool with_vel = s.vel_mirror_wanted && !s.vel_mirrored && s.n > 0;
     if (with_vel) {
-        s.svel.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+        s.svel.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
         if (has_error()) return;
     }
-    hipLaunchKernelGGL(k_mirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
+    hipLaunchKernelGGL(k_mirror, dim3(div_up(bound, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
                        (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
                        s.spos.as<double4>(), s.n, (const double4*)s.vel.as<double4>(),
                        with_vel ? s.svel.as<double4>() : (double4*)nullptr);
@@ -173,6 +212,7 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         int over = 0;
         TMD_HIP(hipMemcpyAsync(&over, counters + 2, sizeof(int), hipMemcpyDeviceToHost, s.stream));
         TMD_HIP(hipStreamSynchronize(s.stream));
+        finish_windows(s);  // statistics of a deferred window build
         if (has_error()) return;
         if (over > 0) {
             s.nb_cap = over * 2;

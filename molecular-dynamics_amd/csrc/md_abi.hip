@@ -243,11 +243,14 @@ static void rebuild(Sim& s, int half, double cutoff) {
     build_ghosts(s, true);  // md_enforce_pbc + borders
     s.timer.end(K_BORDERS, s.stream, ev);
     s.timer.begin(K_BIN, s.stream, &ev);
+    s.defer_window_sync = 1;  // the list build reads the window statistics
     bin_particles(s);
     s.timer.end(K_BIN, s.stream, ev);
     s.timer.begin(K_NEIGHBOR, s.stream, &ev);
     build_neighbor_lists(s, half, cutoff);
     s.timer.end(K_NEIGHBOR, s.stream, ev);
+    s.defer_window_sync = 0;
+    finish_windows(s);  // if the list build stopped early
 }
 
 // ---------------------------------------------------------------------------
@@ -447,6 +450,13 @@ void md_deallocate_grid() {
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
                       &s.adj_acc};
     for (DevBuf* b : bufs) b->release();
+    if (s.h_winstats) {
+        (void)hipHostFree(s.h_winstats);
+        s.h_winstats = nullptr;
+    }
+    s.windows_pending = 0;
+    s.defer_window_sync = 0;
+    s.max_window = 0;
     s.n = s.ng = 0;
     s.cap_all = 0;
     s.nb_valid = 0;

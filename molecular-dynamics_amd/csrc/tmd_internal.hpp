@@ -100,6 +100,9 @@ struct Sim {
     int vel_mirrored = 0;   // local velocities live in svel, in mirror order (fused step loop)
     int vel_mirror_wanted = 0;  // fused step loop: refresh_mirror also moves the velocities to svel
     int gsrc_valid = 0;     // g_ko / g_kg match the ghost table and the binning
+    int defer_window_sync = 0;  // rebuild in progress: build_windows may leave its statistics pending
+    int windows_pending = 0;    // window statistics queued into h_winstats, not yet applied (finish_windows)
+    int* h_winstats = nullptr;  // pinned host copy of the window statistics (4 ints)
     int ghost_pos_stale = 0;  // ghosts in pos lag the mirror (fused step loop; sync_pos_from_mirror)
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
@@ -192,6 +195,7 @@ void build_windows(Sim& s);
 int window_cap(const Sim& s);
 int tile_block_threads(int requested, int max_locals);  // staged window atoms per block (LDS budget, kStageBytes each)
 void refresh_mirror(Sim& s);
+void finish_windows(Sim& s);  // apply window statistics left pending by build_windows
 void build_neighbor_lists(Sim& s, int half, double cutoff);
 
 // forces
