@@ -96,20 +96,6 @@ __global__ void k_b_int1(const RepDesc* __restrict__ R, const int* __restrict__ 
     if (k >= 0) spos[k] = x;
 }
 
-__global__ void k_b_int2(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, double4* __restrict__ vel,
-                         const double4* __restrict__ force, double hdt) {
-    const int r = eb_rep[blockIdx.x];
-    const int i = (blockIdx.x - R[r].eb0) * kBlk + threadIdx.x;
-    if (i >= R[r].n) return;
-    const int s = R[r].a0 + i;
-    double4 v = vel[s];
-    const double4 f = force[s];
-    v.x = v.x + f.x * hdt;
-    v.y = v.y + f.y * hdt;
-    v.z = v.z + f.z * hdt;
-    vel[s] = v;
-}
-
 __global__ void k_b_pbc(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, double4* __restrict__ pos) {
     const int r = eb_rep[blockIdx.x];
     const RepDesc& D = R[r];
