@@ -92,18 +92,25 @@ __device__ __forceinline__ double block_sum_double_any(double v) {
 __device__ __forceinline__ int wave_cell_slot(int c, int* __restrict__ count) {
     const int lane = threadIdx.x & (kWave - 1);
     unsigned long long pending = __ballot(c >= 0);
-    int slot = -1;
+    // group the lanes by cell first (no memory round trip inside the loop),
+    // then every group's leader takes its slots with one atomic, all groups
+    // at once
+    int leader_of = lane;
+    unsigned long long grp_of = 0ull;
     while (pending) {
         const int leader = __ffsll((long long)pending) - 1;
         const int cl = __shfl(c, leader, kWave);
         const unsigned long long grp = __ballot(c == cl) & pending;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[cl], __popcll(grp));
-        base = __shfl(base, leader, kWave);
-        if ((grp >> lane) & 1ull) slot = base + __popcll(grp & ((1ull << lane) - 1ull));
+        if ((grp >> lane) & 1ull) {
+            leader_of = leader;
+            grp_of = grp;
+        }
         pending &= ~grp;
     }
-    return slot;
+    int base = 0;
+    if (c >= 0 && lane == leader_of) base = atomicAdd(&count[c], __popcll(grp_of));
+    base = __shfl(base, leader_of, kWave);
+    return c >= 0 ? base + __popcll(grp_of & ((1ull << lane) - 1ull)) : -1;
 }
 
 // Sort the cell segment parts[b, e) (distinct atom indices) ascending with
