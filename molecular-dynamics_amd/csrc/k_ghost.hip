@@ -83,10 +83,23 @@ __global__ void __launch_bounds__(kScanThreads) k_face_flags(double4* __restrict
         t[4] = ((bits >> 4) & 1) * fy;
         t[5] = ((bits >> 5) & 1) * fy;
     }
+    // block totals of the six counts: wavefront sums by shuffles, then one
+    // LDS pass (integer sums: the order does not matter)
+    __shared__ int wt[kScanThreads / kWave][6];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        const int v = block_sum_int(t[k]);
-        if (threadIdx.x == 0 && v) atomicAdd(&totals[k], v);
+        int v = t[k];
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_down(v, d, kWave);
+        if (lane == 0) wt[wid][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int v = 0;
+#pragma unroll
+        for (int w = 0; w < kScanThreads / kWave; ++w) v += wt[w][threadIdx.x];
+        if (v) atomicAdd(&totals[threadIdx.x], v);
     }
 }
 
