@@ -365,29 +365,36 @@ __global__ void k_b_tile_stats(const RepDesc* __restrict__ R, const int* __restr
                                const int* __restrict__ local_count, int ntiles_all, int* __restrict__ tile_count,
                                int* __restrict__ flags) {
     const int tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= ntiles_all) return;
-    const RepDesc& D = R[tile_rep[tile]];
-    const TileGeom g = D.g;
-    const int* st = start + D.c0;
-    const int* lc = local_count + D.c0;
-    const int lt = tile - D.t0;
-    const int tx_i = lt % g.ntx, ty_i = (lt / g.ntx) % g.nty, tz_i = lt / (g.ntx * g.nty);
-    const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
-    const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
-    int cnt = 0;
-    for (int z = cz0; z < cz1; ++z)
-        for (int y = cy0; y < cy1; ++y)
-            for (int x = cx0; x < cx1; ++x) cnt += lc[(z * g.ny + y) * g.nx + x];
-    tile_count[tile] = cnt;
-    int S = 0;
-    const int xlo = cx0 - 1, xhi = min(cx0 + g.tx, g.nx - 1);
-    for (int z = cz0 - 1; z <= cz0 + g.tz; ++z)
-        for (int y = cy0 - 1; y <= cy0 + g.ty; ++y)
-            if (y < g.ny && z < g.nz) {
-                const int row = (z * g.ny + y) * g.nx;
-                S += st[row + xhi + 1] - st[row + xlo];
-            }
-    if (cnt > 0) {
+    int cnt = 0, S = 0;
+    if (tile < ntiles_all) {
+        const RepDesc& D = R[tile_rep[tile]];
+        const TileGeom g = D.g;
+        const int* st = start + D.c0;
+        const int* lc = local_count + D.c0;
+        const int lt = tile - D.t0;
+        const int tx_i = lt % g.ntx, ty_i = (lt / g.ntx) % g.nty, tz_i = lt / (g.ntx * g.nty);
+        const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+        const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
+        for (int z = cz0; z < cz1; ++z)
+            for (int y = cy0; y < cy1; ++y)
+                for (int x = cx0; x < cx1; ++x) cnt += lc[(z * g.ny + y) * g.nx + x];
+        tile_count[tile] = cnt;
+        const int xlo = cx0 - 1, xhi = min(cx0 + g.tx, g.nx - 1);
+        for (int z = cz0 - 1; z <= cz0 + g.tz; ++z)
+            for (int y = cy0 - 1; y <= cy0 + g.ty; ++y)
+                if (y < g.ny && z < g.nz) {
+                    const int row = (z * g.ny + y) * g.nx;
+                    S += st[row + xhi + 1] - st[row + xlo];
+                }
+        if (cnt == 0) S = 0;  // empty tiles launch no block
+    }
+    // one atomic per wavefront for each maximum
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        S = max(S, __shfl_down(S, d, kWave));
+        cnt = max(cnt, __shfl_down(cnt, d, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && cnt > 0) {
         atomicMax(&flags[FL_MAXWIN], S);
         atomicMax(&flags[FL_MAXLOC], cnt);
         if (S > kWinMax) atomicMax(&flags[FL_WIN], S);

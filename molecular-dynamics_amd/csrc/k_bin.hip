@@ -85,24 +85,31 @@ __global__ void k_tile_stats_small(const int* __restrict__ start, const int* __r
                                    int ntiles, int* __restrict__ tile_count, int* __restrict__ max_window,
                                    int* __restrict__ max_locals) {
     const int tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= ntiles) return;
-    const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
-    const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
-    const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
-    int cnt = 0;
-    for (int z = cz0; z < cz1; ++z)
-        for (int y = cy0; y < cy1; ++y)
-            for (int x = cx0; x < cx1; ++x) cnt += local_count[(z * g.ny + y) * g.nx + x];
-    tile_count[tile] = cnt;
-    int S = 0;
-    const int xlo = cx0 - 1, xhi = min(cx0 + g.tx, g.nx - 1);
-    for (int z = cz0 - 1; z <= cz0 + g.tz; ++z)
-        for (int y = cy0 - 1; y <= cy0 + g.ty; ++y)
-            if (y < g.ny && z < g.nz) {
-                const int row = (z * g.ny + y) * g.nx;
-                S += start[row + xhi + 1] - start[row + xlo];
-            }
-    if (cnt > 0) {
+    int cnt = 0, S = 0;
+    if (tile < ntiles) {
+        const int tx_i = tile % g.ntx, ty_i = (tile / g.ntx) % g.nty, tz_i = tile / (g.ntx * g.nty);
+        const int cx0 = 1 + tx_i * g.tx, cy0 = 1 + ty_i * g.ty, cz0 = 1 + tz_i * g.tz;
+        const int cx1 = min(cx0 + g.tx, g.nx - 1), cy1 = min(cy0 + g.ty, g.ny - 1), cz1 = min(cz0 + g.tz, g.nz - 1);
+        for (int z = cz0; z < cz1; ++z)
+            for (int y = cy0; y < cy1; ++y)
+                for (int x = cx0; x < cx1; ++x) cnt += local_count[(z * g.ny + y) * g.nx + x];
+        tile_count[tile] = cnt;
+        const int xlo = cx0 - 1, xhi = min(cx0 + g.tx, g.nx - 1);
+        for (int z = cz0 - 1; z <= cz0 + g.tz; ++z)
+            for (int y = cy0 - 1; y <= cy0 + g.ty; ++y)
+                if (y < g.ny && z < g.nz) {
+                    const int row = (z * g.ny + y) * g.nx;
+                    S += start[row + xhi + 1] - start[row + xlo];
+                }
+        if (cnt == 0) S = 0;  // empty tiles launch no block
+    }
+    // one atomic per wavefront for each maximum
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        S = max(S, __shfl_down(S, d, kWave));
+        cnt = max(cnt, __shfl_down(cnt, d, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && cnt > 0) {
         atomicMax(max_window, S);
         atomicMax(max_locals, cnt);
     }
