@@ -362,8 +362,8 @@ def main():
         print(json.dumps({"copy_GBps": hbm_copy_gbs()}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU decomposition is design-only "
-              "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: this library runs on one GPU; splitting a box over several GPUs "
+              "is not implemented. Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import tinymd_amd as tm

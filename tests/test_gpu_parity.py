@@ -144,7 +144,7 @@ def test_lj_fp32_mixed_error(tm, half, precision):
     _, _, _, fg = sim.state()
     _, _, fo = orc.state()
     rms, mx = force_error(fg, fo)
-    # <= 1e-5 relative force error vs FP64
+    # FP32-mixed bar: <= 1e-5 relative force error against FP64
     assert rms < 1e-5 and mx < 1e-5 * 10, (rms, mx)
     announce(f"  fp32-mixed rms={rms:.3e} max={mx:.3e}")
 
