@@ -26,14 +26,14 @@ def boxes(tm):
     return tm.replica_boxes(seed=7, nrep=5, cells=(6, 9))
 
 
-def single(tm, box, steps, precision, spacing=1.0, runs=None):
+def single(tm, box, steps, precision, spacing=1.0, runs=None, ren=REN):
     aabb, x, v = box
     sim = tm.Simulation(np.ones(len(x)), x, v, aabb, (RC + SKIN) * spacing, precision=precision)
     sim.copy_to_accelerator()
     sim.setup(False, RC, SKIN)
     sim.lennard_jones(False, RC)
     for s in (runs or [steps]):
-        sim.run_lj(s, REN, DT, False, RC, SKIN)
+        sim.run_lj(s, ren, DT, False, RC, SKIN)
     _, xs, vs, fs = sim.state()
     th = sim.thermo(RC)
     ng = sim.nghost
@@ -41,9 +41,9 @@ def single(tm, box, steps, precision, spacing=1.0, runs=None):
     return xs, vs, fs, th, ng
 
 
-def assert_bitwise(tm, b, boxes, steps, precision, spacing=1.0, runs=None):
+def assert_bitwise(tm, b, boxes, steps, precision, spacing=1.0, runs=None, ren=REN):
     for r, box in enumerate(boxes):
-        xs, vs, fs, _, _ = single(tm, box, steps, precision, spacing, runs)
+        xs, vs, fs, _, _ = single(tm, box, steps, precision, spacing, runs, ren)
         xb, vb, fb = b.state(r)
         assert np.array_equal(xb, xs), f"replica {r} positions"
         assert np.array_equal(vb, vs), f"replica {r} velocities"
@@ -59,6 +59,17 @@ def test_batch_bitwise_vs_single(tm, boxes, precision):
     info = b.info()
     for r, box in enumerate(boxes):
         assert info[r]["ghosts"] == single(tm, box, 45, precision)[4]
+    b.close()
+
+
+def test_batch_short_reneighbour(tm, boxes):
+    """Rebuilds every third step: each rebuild copies the replicas' state
+    back from the step loop's mirrors and rebuilds their ghosts; the
+    replicas stay bit-identical to single runs with the same interval."""
+    announce("batch 5 replicas fp64, rebuild every 3 steps, 14 steps")
+    b = tm.Batch(boxes, precision="fp64", reneigh=3)
+    assert b.run(14) == 14
+    assert_bitwise(tm, b, boxes, 14, "fp64", ren=3)
     b.close()
 
 

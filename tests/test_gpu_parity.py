@@ -219,6 +219,34 @@ def test_fused_loop_ghost_positions(tm, shape):
     sim.close()
 
 
+@pytest.mark.parametrize("reneigh,steps", [(1, 6), (3, 13)])
+def test_trajectory_bitwise_short_reneighbour(tm, reneigh, steps):
+    """Rebuilds every step and every third step of the fused loop.  Each
+    rebuild takes the positions and velocities back from the loop's mirrors
+    in its ghost selection pass, and its list build runs on the previous
+    window statistics.  The FP64 trajectory, the lists and the final ghost
+    positions are the oracle's bit for bit."""
+    announce(f"trajectory 16^3, {steps} steps, rebuild every {reneigh}")
+    aabb, m, x, v = perturbed_fcc(16, amp=0.1, seed=41)
+    sim, orc = make_pair(aabb, m, x, v, threads=8)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    sim.run_lj(steps, reneigh=reneigh, dt=0.005)
+    orc.run_lj(steps, reneigh=reneigh, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    assert sim.nghost == orc.ng
+    _, gx, _, _ = sim.ghost_state()
+    assert np.array_equal(gx, orc.ghosts()["pos"])
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert np.array_equal(cg, co)
+    assert all(np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]) for i in range(0, len(cg), 53))
+    sim.close()
+
+
 def test_half_list_trajectory_close(tm):
     announce("half-list trajectory vs oracle (tolerance)")
     aabb, m, x, v = perturbed_fcc(6, amp=0.05, seed=9)
