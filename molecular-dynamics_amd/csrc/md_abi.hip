@@ -631,7 +631,11 @@ int tmd_device_info(char* buf, int len) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
-    if (buf && len > 0) snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    // some runtimes report no marketing name: the architecture alone then
+    if (buf && len > 0) {
+        if (prop.name[0]) snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+        else snprintf(buf, len, "%s", prop.gcnArchName);
+    }
     return prop.multiProcessorCount;
 }
 
