@@ -15,7 +15,8 @@ extern "C" {
  * 16-bit window overflow), or NULL.  The reference has no error channel. */
 const char* tmd_last_error(void);
 void tmd_clear_error(void);
-/* Device name into buf; returns the number of compute units (0 = no GPU). */
+/* Device name and architecture (the architecture alone when the runtime
+ * reports no name) into buf; returns the number of compute units (0 = no GPU). */
 int tmd_device_info(char* buf, int len);
 
 /* ---- modes ---------------------------------------------------------------- */
