@@ -803,6 +803,8 @@ static void layout(Batch& B) {
     B.epart.reserve(sizeof(double) * 3 * (size_t)B.n_tiles, false, st);
     B.kpart.reserve(sizeof(double) * (size_t)B.n_eb, false, st);
     B.tout.reserve(sizeof(double) * 4 * (size_t)B.nrep, false, st);
+    // the per-block tables above were uploaded from local (pageable) vectors
+    TMD_HIP(hipStreamSynchronize(st));
 }
 
 static void reserve_lists(Batch& B) {
