@@ -185,12 +185,18 @@ __device__ __forceinline__ void lj_energy_atom(const double4 pi, int cnt, const 
                                                const TileWin& W, const LJArgs& a, int half, int nloc_end, double& pe,
                                                double& vir, double& np) {
     const int ng = (cnt + 7) >> 3;
+    // the next group of 8 entries is loaded while the current one is used
+    int4 q = ld_stream(lp);  // group 0 of the column always exists (capacity >= 8)
     for (int gi = 0; gi < ng; ++gi) {
-        const int4 e = lp[(size_t)gi * 64];
+        const int4 e = q;
+        q = ld_stream(lp + (size_t)min(gi + 1, ng - 1) * 64);  // next group (clamped: always a valid load)
         const int kend = cnt - gi * 8;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            if (u >= kend) break;
+            // staged, the entries past the count are the sentinel slot (far
+            // away, j = -1) and fail the cutoff test; from the mirror they
+            // would index past the window
+            if (!STAGED && u >= kend) break;
             const int t = nb_entry(e, u);
             int j;
             double xj, yj, zj;
