@@ -48,11 +48,11 @@ void md_rescale_grid(double const *);
 void md_copy_data_from_accelerator(void);
 /* core/runtime.impala:104, core/grid.impala:360-367 */
 void md_copy_data_to_accelerator(void);
-/* core/runtime.impala:106, core/compute.impala:175-184 */
+/* core/runtime.impala:106, core/compute.impala:97-106 */
 void md_initial_integration(double);
-/* core/runtime.impala:107, core/compute.impala:186-194 */
+/* core/runtime.impala:107, core/compute.impala:108-116 */
 void md_final_integration(double);
-/* core/runtime.impala:108, core/compute.impala:196-200 */
+/* core/runtime.impala:108, core/compute.impala:118-122 */
 void md_enforce_pbc(void);
 /* core/runtime.impala:110-117, core/grid.impala:483-499 */
 int md_write_grid_data_to_arrays(double *, struct Vector3D *, struct Vector3D *, struct Vector3D *);

@@ -18,6 +18,10 @@ void tmd_clear_error(void);
 /* Device name and architecture (the architecture alone when the runtime
  * reports no name) into buf; returns the number of compute units (0 = no GPU). */
 int tmd_device_info(char* buf, int len);
+/* Host evaluation (no GPU needed) of the compensated FP32 sum that the
+ * FP32-Kahan pair kernels apply to their per-atom force terms: the same
+ * inline accumulation and finalisation, returned in double. */
+double tmd_kahan_sum(const float* x, int n);
 
 /* ---- modes ---------------------------------------------------------------- */
 /* 0 = FP64 (default; the reference's real_t, utils/utilities.impala:1),
@@ -155,6 +159,10 @@ int tmd_batch_configure(int handle, int tile, int threads, double spacing_factor
 int tmd_batch_autotune(int handle, int trial_steps, double* out5);
 /* Restart from the initial state with the current configuration. */
 int tmd_batch_reset(int handle);
+/* 1 (default): a run is captured once into a hipGraph and launched as one
+ * graph; 0: the same kernels are launched one by one on the batch stream
+ * (for tools that trace individual launches).  Results are identical. */
+int tmd_batch_set_graph(int handle, int on);
 /* Advance all replicas `steps` steps (one graph launch); returns steps.
  * The timed variant also returns the GPU time of the launch. */
 int tmd_batch_run(int handle, int steps);

@@ -659,6 +659,7 @@ struct Batch {
         std::vector<unsigned long long> sig;  // pointers and scalars baked into the graph
     };
     std::map<int, CapturedRun> graphs;
+    bool use_graph = true;  // false: a run's kernels are enqueued directly (no capture)
     bool ready = false;
 
     ~Batch() {
@@ -896,18 +897,34 @@ static void enq_unmirror(Batch& B, const DevBuf& m) {
                        B.pos.as<double4>());
 }
 
+// LDS layout of the list-build launch: the row starts of the largest tile
+// shape among the replicas (rsb bytes) after the staged window, whose atom
+// capacity is what the budget leaves.  These values are baked into a
+// captured run, so graph_signature includes them.
+struct NlistLds {
+    int rsb, wcap;
+    size_t bytes;
+};
+
+static NlistLds nlist_lds(const Batch& B) {
+    NlistLds l;
+    l.rsb = 0;
+    for (const RepDesc& D : B.R) l.rsb = std::max(l.rsb, row_starts_bytes(D.g));
+    l.wcap = std::min(B.wcap, (kLdsBudget - l.rsb) / kStageBytes);
+    l.bytes = (size_t)l.wcap * kStageBytes + l.rsb;
+    return l;
+}
+
 static void enq_nlist(Batch& B, const DevBuf& m) {
     NbArgs a{};
     a.cap = B.list_cap;
     a.half = 0;
     a.half_mode = HALF_GHOST_FULL;
-    // row starts of the largest tile shape among the replicas
-    int rsb = 0;
-    for (const RepDesc& D : B.R) rsb = std::max(rsb, row_starts_bytes(D.g));
-    a.wcap = std::min(B.wcap, (kLdsBudget - rsb) / kStageBytes);
+    const NlistLds l = nlist_lds(B);
+    a.wcap = l.wcap;
     const double cut = B.rc + B.skin;
     a.cut2 = cut * cut;
-    const size_t lds = (size_t)a.wcap * kStageBytes + rsb;
+    const size_t lds = l.bytes;
     hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.tile_rep.as<int>(), (const double4*)m.as<double4>(), (const int*)B.work_k.as<int>(),
                        (const int*)B.cell_of.as<int>(), (const int*)B.cell_start.as<int>(),
@@ -1018,6 +1035,33 @@ static void upload_initial(Batch& B) {
     TMD_HIP(hipStreamSynchronize(B.stream));
 }
 
+// Grow every replica's ghost room (a ghost overflow), re-lay out the
+// buffers and carry the locals' positions and velocities over to their new
+// slots.  Every copy is ordered on the batch stream and waited for: the
+// read-back must see the work queued before it, and the re-laid-out
+// buffers must hold the state before anything is enqueued behind it.
+static void grow_ghost_room(Batch& B) {
+    std::vector<double4> p((size_t)B.n_slots), v((size_t)B.n_slots);
+    TMD_HIP(hipMemcpyAsync(p.data(), B.pos.p, sizeof(double4) * p.size(), hipMemcpyDeviceToHost, B.stream));
+    TMD_HIP(hipMemcpyAsync(v.data(), B.vel.p, sizeof(double4) * v.size(), hipMemcpyDeviceToHost, B.stream));
+    TMD_HIP(hipStreamSynchronize(B.stream));
+    if (has_error()) return;
+    const std::vector<RepDesc> old = B.R;
+    B.ghost_scale *= 1.6;
+    B.drop_graphs();  // every captured run refers to the old buffers and tables
+    layout(B);
+    if (has_error()) return;
+    std::vector<double4> p2((size_t)B.n_slots, make_double4(0, 0, 0, 0)), v2((size_t)B.n_slots, make_double4(0, 0, 0, 0));
+    for (int r = 0; r < B.nrep; ++r)
+        for (int i = 0; i < B.n[r]; ++i) {
+            p2[(size_t)B.R[r].a0 + i] = p[(size_t)old[r].a0 + i];
+            v2[(size_t)B.R[r].a0 + i] = v[(size_t)old[r].a0 + i];
+        }
+    TMD_HIP(hipMemcpyAsync(B.pos.p, p2.data(), sizeof(double4) * p2.size(), hipMemcpyHostToDevice, B.stream));
+    TMD_HIP(hipMemcpyAsync(B.vel.p, v2.data(), sizeof(double4) * v2.size(), hipMemcpyHostToDevice, B.stream));
+    TMD_HIP(hipStreamSynchronize(B.stream));
+}
+
 // Initial rebuild + force at the current positions, sizing the ghost room,
 // the LDS window and the list capacity.  Host synchronisation allowed.
 static bool setup_lists(Batch& B) {
@@ -1029,21 +1073,8 @@ static bool setup_lists(Batch& B) {
         if (has_error()) return false;
         if (fl[FL_GHOST] > 0) {
             // ghost room too small: grow, re-lay out, restore the state
-            std::vector<double4> p((size_t)B.n_slots), v((size_t)B.n_slots);
-            TMD_HIP(hipMemcpy(p.data(), B.pos.p, sizeof(double4) * p.size(), hipMemcpyDeviceToHost));
-            TMD_HIP(hipMemcpy(v.data(), B.vel.p, sizeof(double4) * v.size(), hipMemcpyDeviceToHost));
-            const std::vector<RepDesc> old = B.R;
-            B.ghost_scale *= 1.6;
-            layout(B);
+            grow_ghost_room(B);
             if (has_error()) return false;
-            std::vector<double4> p2((size_t)B.n_slots), v2((size_t)B.n_slots);
-            for (int r = 0; r < B.nrep; ++r)
-                for (int i = 0; i < B.n[r]; ++i) {
-                    p2[(size_t)B.R[r].a0 + i] = p[(size_t)old[r].a0 + i];
-                    v2[(size_t)B.R[r].a0 + i] = v[(size_t)old[r].a0 + i];
-                }
-            TMD_HIP(hipMemcpy(B.pos.p, p2.data(), sizeof(double4) * p2.size(), hipMemcpyHostToDevice));
-            TMD_HIP(hipMemcpy(B.vel.p, v2.data(), sizeof(double4) * v2.size(), hipMemcpyHostToDevice));
             continue;
         }
         if (fl[FL_UNBINNED] > 0) {
@@ -1109,9 +1140,16 @@ static std::vector<unsigned long long> graph_signature(const Batch& B) {
         std::memcpy(&u, &d, sizeof u);
         s.push_back(u);
     }
+    const NlistLds l = nlist_lds(B);
     const int iv[] = {B.reneigh, B.precision, B.shift, B.list_cap, B.wcap, B.threads, B.n_sb, B.n_eb, B.n_cb, B.n_gb,
-                      B.n_tiles, B.n_cells, B.n_slots, B.nrep};
+                      B.n_tiles, B.n_cells, B.n_slots, B.nrep, l.rsb, l.wcap, (int)l.bytes};
     for (int v : iv) s.push_back((unsigned long long)(unsigned)v);
+    // the host-side tile shapes behind the launch sizes above: two layouts
+    // with equal totals but different tile edges must not share a graph
+    for (const RepDesc& D : B.R) {
+        const int gv[] = {D.g.nx, D.g.ny, D.g.nz, D.g.tx, D.g.ty, D.g.tz, D.g.ntx, D.g.nty, D.g.ntz};
+        for (int v : gv) s.push_back((unsigned long long)(unsigned)v);
+    }
     return s;
 }
 
@@ -1158,15 +1196,19 @@ static bool run_steps(Batch& B, int steps, double* gpu_ms) {
     for (int attempt = 0; attempt < 6 && !has_error(); ++attempt) {
         copy_all(B, B.snap_pos, B.snap_vel, B.snap_force, B.pos, B.vel, B.force);
         fill(B, B.flags, FL_N, 0);
-        hipGraphExec_t ge = get_graph(B, steps);
-        if (!ge) return false;
+        hipGraphExec_t ge = nullptr;
+        if (B.use_graph) {
+            ge = get_graph(B, steps);
+            if (!ge) return false;
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (gpu_ms) {
             TMD_HIP(hipEventCreate(&e0));
             TMD_HIP(hipEventCreate(&e1));
             TMD_HIP(hipEventRecord(e0, B.stream));
         }
-        TMD_HIP(hipGraphLaunch(ge, B.stream));
+        if (ge) TMD_HIP(hipGraphLaunch(ge, B.stream));
+        else enq_steps(B, steps);  // the same kernels, launched one by one
         if (gpu_ms) TMD_HIP(hipEventRecord(e1, B.stream));
         int fl[FL_N];
         read_flags(B, fl);
@@ -1187,27 +1229,19 @@ static bool run_steps(Batch& B, int steps, double* gpu_ms) {
             return false;
         }
         if (fl[FL_GHOST] == 0 && fl[FL_LIST] == 0) return true;
-        // overflow: restore the starting state, grow, rebuild there, rerun
+        // overflow: restore the starting state, grow, rebuild there, rerun.
+        // The restore copies run on the batch stream, which does not
+        // synchronise with the null stream: wait for them before the state
+        // is read back (grow_ghost_room) or rebuilt.
         copy_all(B, B.pos, B.vel, B.force, B.snap_pos, B.snap_vel, B.snap_force);
+        TMD_HIP(hipStreamSynchronize(B.stream));
         if (fl[FL_LIST] > 0) B.list_cap = ((int)std::ceil(fl[FL_LIST] * B.cfg.list_headroom) + 7) & ~7;
         B.drop_graphs();
         if (fl[FL_GHOST] > 0) {
-            std::vector<double4> p((size_t)B.n_slots), v((size_t)B.n_slots);
-            TMD_HIP(hipMemcpy(p.data(), B.pos.p, sizeof(double4) * p.size(), hipMemcpyDeviceToHost));
-            TMD_HIP(hipMemcpy(v.data(), B.vel.p, sizeof(double4) * v.size(), hipMemcpyDeviceToHost));
-            const std::vector<RepDesc> old = B.R;
-            B.ghost_scale *= 1.6;
             const int keep_cap = B.list_cap;
-            layout(B);
+            grow_ghost_room(B);
             B.list_cap = keep_cap;
-            std::vector<double4> p2((size_t)B.n_slots), v2((size_t)B.n_slots);
-            for (int r = 0; r < B.nrep; ++r)
-                for (int i = 0; i < B.n[r]; ++i) {
-                    p2[(size_t)B.R[r].a0 + i] = p[(size_t)old[r].a0 + i];
-                    v2[(size_t)B.R[r].a0 + i] = v[(size_t)old[r].a0 + i];
-                }
-            TMD_HIP(hipMemcpy(B.pos.p, p2.data(), sizeof(double4) * p2.size(), hipMemcpyHostToDevice));
-            TMD_HIP(hipMemcpy(B.vel.p, v2.data(), sizeof(double4) * v2.size(), hipMemcpyHostToDevice));
+            if (has_error()) return false;
         }
         if (!setup_lists(B)) return false;
     }
@@ -1354,6 +1388,14 @@ int tmd_batch_configure(int h, int tile, int threads, double spacing_factor, dou
     B->cfg.ghost_headroom = ghost_headroom;
     B->ghost_scale = 1.0;
     return setup_from_initial(*B) ? 1 : 0;
+}
+
+int tmd_batch_set_graph(int h, int on) {
+    Batch* B = find_batch(h);
+    if (!B || has_error()) return 0;
+    B->use_graph = on != 0;
+    if (!B->use_graph) B->drop_graphs();
+    return 1;
 }
 
 int tmd_batch_run(int h, int steps) {

@@ -9,7 +9,7 @@
 // 2 bytes of list per entry from HBM and does no global gathers.
 //
 // Full lists accumulate the force of lane i in registers and store it once:
-// no atomics (core/compute.impala:92-123, full branch).
+// no atomics (core/compute.impala:14-45, full branch).
 //
 // Half lists (each pair once in the list) are evaluated by gather as well:
 // the list build also writes each atom's complementary column (its local
@@ -319,6 +319,7 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
     LJArgs a = make_lj_args(s, rc, eps, sigma);
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     const int shift = s.lj_shift;
+    s.last_force_rc = rc;
     hipEvent_t ev = nullptr;
     s.timer.begin(K_MIRROR, s.stream, &ev);
     if (!s.mirror_fresh) refresh_mirror(s);
@@ -379,6 +380,7 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
     if (has_error() || s.n == 0) return;
     const double radius = s.dem_radius > 0.0 ? s.dem_radius : 0.5 * rc;
+    s.last_force_rc = rc;
     DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n, window_cap(s), s.tiles};
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     const double4* V = s.vel.as<double4>();
@@ -443,3 +445,12 @@ void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, d
 }
 
 }  // namespace tmd
+
+// Host evaluation of the compensated sum that the FP32-Kahan pair kernels
+// use (kahan_add / kahan_value, the same inline functions), so that the
+// finalisation can be checked without a GPU.
+extern "C" double tmd_kahan_sum(const float* x, int n) {
+    float s = 0.f, c = 0.f;
+    for (int k = 0; k < n; ++k) tmd::kahan_add(s, c, x[k]);
+    return tmd::kahan_value(s, c);
+}

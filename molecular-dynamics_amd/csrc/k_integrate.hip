@@ -1,5 +1,5 @@
 // Velocity Verlet (unit mass), the periodic wrap and the kinetic-energy
-// reduction.  core/compute.impala:175-200 (integration, PBC),
+// reduction.  core/compute.impala:97-122 (integration, PBC),
 // comm/comm.impala:152-162 (single-image wrap), core/thermo.impala:1-16.
 #include "tmd_internal.hpp"
 #include "tmd_device.hpp"

@@ -583,16 +583,30 @@ void md_compute_boundary_weights(double xmin, double xmax, double ymin, double y
 
 void md_update_neighborhood(int, int, int const*, int const*, double const*) {}
 
+// ForceIters = list entries visited by the last pair-force evaluation,
+// RsqIters = those inside its cutoff, IntegrationIters = atoms of the last
+// integration (core/compute.impala:1-48, 97-116; scalar counts, one per
+// entry and atom).  Both list counts are taken from the current list and
+// positions, which are those of the last force evaluation: the final
+// integration moves only velocities.
 void md_report_iterations() {
     Sim& s = sim();
-    long long entries = 0;
+    long long entries = 0, in_cut = 0;
     if (s.nb_valid && s.stream && !has_error()) {
         std::vector<int> c(std::max(s.n, 1));
-        TMD_HIP(hipMemcpy(c.data(), s.nb_count.p, sizeof(int) * s.n, hipMemcpyDeviceToHost));
+        TMD_HIP(hipMemcpyAsync(c.data(), s.nb_count.p, sizeof(int) * s.n, hipMemcpyDeviceToHost, s.stream));
+        TMD_HIP(hipStreamSynchronize(s.stream));
         for (int i = 0; i < s.n; ++i) entries += c[i];
+        if (s.last_force_rc > 0.0) {
+            if (s.pos_stale || s.ghost_stale) sync_pos_from_mirror(s);
+            double pe = 0.0, vir = 0.0;
+            lj_energy_virial(s, s.last_force_rc, 1.0, 1.0, &pe, &vir, &in_cut);
+        }
     }
+    s.force_iters = entries;
+    s.rsq_iters = in_cut;
     printf("ForceIters RsqIters IntegrationIters\n");
-    printf("%lld %lld %lld \n", entries, s.pairs, s.integration_iters);
+    printf("%lld %lld %lld \n", entries, in_cut, s.integration_iters);
     fflush(stdout);
 }
 

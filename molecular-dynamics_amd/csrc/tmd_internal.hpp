@@ -150,6 +150,7 @@ struct Sim {
     long long pairs = 0;
     // iteration counters reported by md_report_iterations
     long long force_iters = 0, rsq_iters = 0, integration_iters = 0;
+    double last_force_rc = 0.0;  // cutoff of the last pair-force evaluation (0: none yet)
 };
 
 // Positions changed outside the fused step loop: the mirror is rebuilt from

@@ -55,13 +55,18 @@ __device__ __forceinline__ float lj_f32u(const LJArgs& a, float rsq) {
     return f;
 }
 
-// Kahan compensated summation (s, c) += x
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+// Kahan compensated summation (s, c) += x.  c keeps the excess that the
+// rounded addition put into s, (t - s) - y, so the compensated value of the
+// sum is s - c (kahan_value).  The translation unit is built without FP
+// contraction, so the four operations are not fused.
+__host__ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
     const float y = x - c;
     const float t = s + y;
     c = (t - s) - y;
     s = t;
 }
+
+__host__ __device__ __forceinline__ double kahan_value(float s, float c) { return (double)s - (double)c; }
 
 // Neighbour position of window offset t: LDS (x, y, z interleaved) when the
 // window is staged, else the cell-ordered mirror.
@@ -170,7 +175,7 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
         }
     }
     if (PREC != PREC_FP32_KAHAN) return make_double4(fx, fy, fz, 0.0);
-    return make_double4((double)sxa + (double)cxa, (double)sya + (double)cya, (double)sza + (double)cza, 0.0);
+    return make_double4(kahan_value(sxa, cxa), kahan_value(sya, cya), kahan_value(sza, cza), 0.0);
 }
 
 

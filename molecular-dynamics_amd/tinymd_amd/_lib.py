@@ -72,6 +72,7 @@ SIGNATURES = {
     "tmd_last_error": (ctypes.c_char_p, []),
     "tmd_clear_error": (None, []),
     "tmd_device_info": (_i, [ctypes.c_char_p, _i]),
+    "tmd_kahan_sum": (_d, [ctypes.POINTER(ctypes.c_float), _i]),
     "tmd_set_precision": (None, [_i]),
     "tmd_set_half_mode": (None, [_i]),
     "tmd_set_ghost_mode": (None, [_i]),
@@ -98,6 +99,7 @@ SIGNATURES = {
     "tmd_batch_configure": (_i, [_i, _i, _i, _d, _d, _d]),
     "tmd_batch_autotune": (_i, [_i, _i, _dp]),
     "tmd_batch_reset": (_i, [_i]),
+    "tmd_batch_set_graph": (_i, [_i, _i]),
     "tmd_batch_run": (_i, [_i, _i]),
     "tmd_batch_run_timed": (_i, [_i, _i, _dp]),
     "tmd_batch_get_state": (_i, [_i, _i, _vp, _vp, _vp]),

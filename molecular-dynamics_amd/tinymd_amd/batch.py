@@ -101,6 +101,12 @@ class Batch:
         self.L.tmd_batch_reset(self.h)
         check()
 
+    def set_graph(self, on=True):
+        """True (default): each run is one hipGraph launch; False: the same
+        kernels are launched one by one (for per-launch tracing tools)."""
+        self.L.tmd_batch_set_graph(self.h, int(bool(on)))
+        check()
+
     def run(self, steps):
         done = self.L.tmd_batch_run(self.h, steps)
         check()

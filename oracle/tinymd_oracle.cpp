@@ -404,7 +404,7 @@ inline double lj_pair(const LJ& p, double rsq, double* u, double* w) {
 
 // compute_potential driver: full lists accumulate in registers and store
 // once; half lists zero the forces and add +F to i and -F to local j.
-// core/compute.impala:79-126.
+// core/compute.impala:1-48.
 void compute_lj(Sim* s, int half, const LJ& p) {
     const int n = s->n;
     s->f.assign(3 * (size_t)n, 0.0);
@@ -518,7 +518,7 @@ void compute_dem(Sim* s, int half, const DEM& p) {
     s->pe = 0.0; s->virial = 0.0; s->pairs_in_cutoff = pairs;
 }
 
-// Velocity Verlet, unit mass.  core/compute.impala:175-194
+// Velocity Verlet, unit mass.  core/compute.impala:97-116
 void initial_integration(Sim* s, double dt) {
     for (int i = 0; i < s->n; ++i)
         for (int e = 0; e < 3; ++e) {

@@ -49,3 +49,21 @@ def force_error(f, fref):
     rms = np.sqrt(np.mean(np.sum(fref * fref, axis=1)))
     d = f - fref
     return np.sqrt(np.mean(np.sum(d * d, axis=1))) / rms, np.max(np.abs(d)) / rms
+
+
+def force_error_atom(f, fref, floor=None):
+    """Per-atom relative force error |F_i - F_ref,i| / max(|F_ref,i|, floor),
+    returned as (rms over atoms, max over atoms).  The floor defaults to the
+    RMS of |F_ref,i| over the atoms, so an atom whose force nearly cancels is
+    measured against the typical force rather than against its own
+    near-zero magnitude."""
+    fn = np.sqrt(np.sum(fref * fref, axis=1))
+    if floor is None:
+        floor = np.sqrt(np.mean(fn * fn))
+    d = np.sqrt(np.sum((f - fref) ** 2, axis=1)) / np.maximum(fn, floor)
+    return float(np.sqrt(np.mean(d * d))), float(np.max(d))
+
+
+# the FP32-mixed bar against FP64: per-atom relative force error (floor: the
+# RMS force) at most 1e-5, both as RMS and as maximum over the atoms
+FP32_BAR = 1e-5

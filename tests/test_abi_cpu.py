@@ -151,3 +151,56 @@ def test_replica_set_generator():
         assert aabb[1] == pytest.approx(nc * (4.0 / rho) ** (1.0 / 3.0))
         assert np.abs(v.mean(axis=0)).max() < 1e-12
         assert np.sum(v * v) / (3 * len(v) - 3) == pytest.approx(temp, rel=1e-12)
+
+
+def test_kahan_finalisation():
+    """The compensated FP32 sum of the FP32-Kahan pair kernels (the library's
+    own inline helpers, evaluated on the host).  Kahan's bound: the error of
+    the compensated sum is at most (2u + O(n u^2)) sum |x_i| with u = 2^-24,
+    whatever n; the plain FP32 running sum has no such bound and is worse on
+    average.  The exact sum of the FP32 inputs is math.fsum."""
+    import math
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+    import numpy as np
+    import tinymd_amd as tm
+    L = tm.load()
+    rng = np.random.default_rng(5)
+    u = 2.0 ** -24
+    err_k, err_n, err_wrong = [], [], []
+    for trial in range(200):
+        n = 80
+        # pair-force-like terms: mixed signs and magnitudes, large cancellation
+        x = (rng.standard_normal(n) * np.exp(rng.uniform(-3, 4, n))).astype(np.float32)
+        exact = math.fsum(float(t) for t in x)
+        got = L.tmd_kahan_sum(x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n)
+        bound = (2 * u + 4 * n * u * u) * float(np.abs(x.astype(np.float64)).sum())
+        assert abs(got - exact) <= bound, (trial, got, exact, bound)
+        err_k.append(abs(got - exact))
+        s = np.float32(0.0)
+        c = np.float32(0.0)
+        for t in x:  # negative control: the finalisation with the sign flipped
+            y = np.float32(t - c)
+            tt = np.float32(s + y)
+            c = np.float32(np.float32(tt - s) - y)
+            s = tt
+        err_wrong.append(abs((float(s) + float(c)) - exact))
+        err_n.append(abs(float(np.cumsum(x, dtype=np.float32)[-1]) - exact))
+    assert np.mean(err_k) < 0.25 * np.mean(err_n), (np.mean(err_k), np.mean(err_n))
+    # the flipped sign adds the rounding error a second time instead of
+    # removing it: about twice the compensated error
+    assert np.mean(err_wrong) > 1.5 * np.mean(err_k), (np.mean(err_wrong), np.mean(err_k))
+
+
+def test_report_time_format(capfd):
+    """md_report_time prints the reference's header and the six values with
+    a trailing blank (core/runtime.impala:318-337; the layout of the
+    results logs, e.g. results/single_gpu_v100.txt:96-97)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
+    import tinymd_amd as tm
+    L = tm.load()
+    capfd.readouterr()
+    L.md_report_time(154.0, 84.0829, 51.3265, 19.8786, 0.0, 0.296124)
+    out = capfd.readouterr().out
+    assert out == "Total Force Neigh Comm Balance Other\n154 84.0829 51.3265 19.8786 0 0.296124 \n", repr(out)

@@ -136,3 +136,59 @@ def test_batch_autotune_keeps_results(tm, boxes):
     b.run(45)
     assert_bitwise(tm, b, boxes, 45, "fp32mixed", spacing=cfg["spacing"])
     b.close()
+
+
+def test_batch_graph_reuse_across_tile_edges(tm, boxes):
+    """A run captured under one tile edge is not reused under another: the
+    launch shapes and LDS sizes a graph bakes in follow the configuration.
+    configure(tile 3) run, configure(tile 4) run: bit-identical to a fresh
+    batch configured with tile 4."""
+    announce("batch graph reuse: tile 3 run, tile 4 run vs fresh tile 4")
+    b = tm.Batch(boxes, precision="fp64")
+    b.configure(tile=3)
+    b.run(25)
+    b.configure(tile=4)
+    b.run(25)
+    f = tm.Batch(boxes, precision="fp64")
+    f.configure(tile=4)
+    f.run(25)
+    for r in range(len(boxes)):
+        for a, c in zip(b.state(r), f.state(r)):
+            assert np.array_equal(a, c), r
+    b.close()
+    f.close()
+
+
+def expanding_box(n_side=9, spacing=1.25, L=22.4, rate=9.0):
+    """A simple-cubic block of atoms in the middle of a periodic box, with
+    velocities pointing away from the centre (v = rate * (x - c)): no atom is
+    within a cell spacing of a face at t = 0 (no ghosts), but after 20 steps
+    the outer layers are, so the ghost room sized at set-up overflows at the
+    first rebuild inside the run."""
+    g = (np.arange(n_side) - (n_side - 1) / 2.0) * spacing
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    d = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    c = L / 2.0
+    return np.array([0.0, L, 0.0, L, 0.0, L]), c + d, rate * d
+
+
+def test_batch_ghost_overflow_inside_run(tm):
+    """Ghost room exhausted at the step-20 rebuild of a run: the run restores
+    its starting state, grows the room, rebuilds and runs again.  The result
+    is bit-identical to the single-box path, and the room grew."""
+    announce("batch ghost overflow at a rebuild inside the run (expanding block)")
+    box = expanding_box()
+    box2 = expanding_box(n_side=8, spacing=1.3)
+    b = tm.Batch([box, box2], precision="fp64")
+    b.configure(ghost_headroom=0.25)
+    room0 = [i["ghost_room"] for i in b.info()]
+    assert all(i["ghosts"] == 0 for i in b.info())
+    assert b.run(25) == 25
+    info = b.info()
+    assert any(i["ghost_room"] > r0 for i, r0 in zip(info, room0)), (room0, info)
+    assert all(i["ghost_room"] >= i["ghosts"] for i in info)
+    for r, bx in enumerate((box, box2)):
+        xs, vs, fs, _, _ = single(tm, bx, 25, "fp64")
+        xb, vb, fb = b.state(r)
+        assert np.array_equal(xb, xs) and np.array_equal(vb, vs) and np.array_equal(fb, fs), r
+    b.close()

@@ -194,6 +194,45 @@ def test_ghosts_brute_force(shape):
     assert np.array_equal(g["pos"], x[g["owner"]] + g["img"] * L)
 
 
+@pytest.mark.parametrize("mode", [po.GHOST_REF_LAG, po.GHOST_FRESH])
+def test_ghost_refresh_lag_closed_form(mode):
+    """Per-step ghost refresh with every local moving by the same constant
+    displacement d per step (no forces; the ghost table fixed at t = 0).
+
+    The reference refreshes the ghost layer in two phases over the ghost
+    table built by borders(): first every ghost's value is packed from its
+    immediate source (a local, or a ghost of an earlier dimension) as it
+    stands, then all ghosts are unpacked (comm/comm.impala:341-387).  A
+    ghost whose source is a local is therefore current after refresh t; a
+    ghost of a ghost reads its source's value of refresh t - 1, and a ghost
+    at chain depth l (1 = its source is a local) lags l - 1 refreshes:
+        x_g(t) = x0[owner] + image L + max(0, t - l + 1) d.
+    The fresh refresh puts every ghost at owner + image L, so there
+    x_g(t) = x0[owner] + image L + t d at any depth."""
+    aabb, m, x, v = po.lattice(4, 4, 4)
+    rng = np.random.default_rng(17)
+    L = aabb[1::2] - aabb[0::2]
+    x0 = np.mod(x + rng.uniform(-0.2, 0.2, x.shape), L)
+    s = po.OracleSim(m, x0, v, aabb, RC + SKIN)
+    s.set_ghost_mode(mode)
+    s.setup(False, po.HALF_GHOST_FULL, RC + SKIN)
+    g = s.ghosts()
+    n = len(x0)
+    depth = np.zeros(len(g["src"]), dtype=int)
+    for k, src in enumerate(g["src"]):  # sources precede their ghosts
+        depth[k] = 1 if src < n else depth[src - n] + 1
+    assert depth.max() == 3  # corner ghosts (three chained images) exist in a 4^3 box
+    d = np.array([0.0110, -0.0070, 0.0045])
+    for t in range(1, 5):
+        s.set_state(x=x0 + t * d)
+        s.sync_ghosts()
+        lag = np.maximum(0, t - depth + 1) if mode == po.GHOST_REF_LAG else np.full(len(depth), t)
+        expect = x0[g["owner"]] + g["img"] * L + lag[:, None] * d
+        got = s.ghosts()["pos"]
+        assert np.abs(got - expect).max() < 1e-12, (t, np.abs(got - expect).max())
+    s.close()
+
+
 def test_ghost_counts_known_sizes():
     for nc, expect in ((16, 13275), (32, 47883)):
         aabb, m, x, v = po.lattice(nc, nc, nc)
