@@ -28,47 +28,74 @@ sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
 METRIC = "LJ atom-steps/s, FP32-mixed vs FP64 at ≤1e-5 force err; adjoint/fwd cost"
 NC = 80
 RC, SKIN, DT, REN = 2.5, 0.3, 0.005, 20
 
 
-def force_kernel_bytes(sim, precision):
-    """Algorithmic HBM bytes of one step-loop pair-force launch (full list,
-    kick and drift fused): neighbour count, atom index and mirror index of
-    each work slot (4 B each per atom), 4 B per list entry (the reference's
-    i32 list; this build stores 2 B), each position read once (24 B, x/y/z
-    FP64), each local's new position written once (24 B; the force of a step
-    that is followed by another stays in registers), each velocity read and
-    written once by the kicks (48 B)."""
+def force_kernel_model(sim, rc=RC):
+    """Algorithmic bytes and flops of one full-list pair-force launch at the
+    current state, by the compulsory-traffic model of the workload: each
+    array read once, each output written once, neighbour gathers assumed
+    cache-resident after their first touch.
+
+      bytes = 4 N (list counts) + 4 entries (4-byte list entries, the
+              reference's i32 list) + 24 N_all (FP64 positions, locals and
+              ghosts, once) + 24 N (one FP64 3-vector out per atom)
+      flops = 23 per in-cutoff entry (del 3, rsq 5, 1/rsq 1, sr6 3, scale 5,
+              accumulate 6; an FMA counts 2) + 8 per entry in the skin
+              (rc < r <= rc + skin: del and rsq)
+
+    Also returned: the same bytes with the 2-byte entries this library
+    stores, and a model of the fused step-loop launch (the kick and drift
+    read and write the velocity, 48 N, and each work slot reads three 4-byte
+    words, 12 N, instead of the count alone)."""
     counts, _ = sim.neighbor_lists(with_lists=False)
     n = sim.nlocal
     nall = n + sim.nghost
     entries = int(counts.astype(np.int64).sum())
-    return 3 * 4 * n + 4 * entries + 24 * nall + 24 * n + 48 * n, entries
+    in_cut = int(sim.thermo(rc)["pairs"])
+    b = 4 * n + 4 * entries + 24 * nall + 24 * n
+    return {"atoms": n, "atoms_with_ghosts": nall, "list_entries": entries, "in_cutoff_entries": in_cut,
+            "bytes": b, "bytes_2B_entries": b - 2 * entries,
+            "bytes_fused_step_model": 12 * n + 4 * entries + 24 * nall + 24 * n + 48 * n,
+            "flops": 23 * in_cut + 8 * (entries - in_cut)}
+
+
+# the reference's phase timers (main.cpp:716-762): Force includes both
+# integration half-steps, Neigh the list build, Comm the ghost borders and
+# per-step ghost refresh, Other the PBC wrap, binning and (here) the
+# cell-ordered mirror
+PHASES = {"force": ("force", "integrate1", "integrate2"), "neigh": ("neighbor",),
+          "comm": ("borders", "ghost_sync"), "other": ("bin", "mirror")}
 
 
 def run_variant(tm, precision, half, steps, warmup, profile=False):
     """Timed run of `steps` steps (no per-kernel timers), then, with
     profile, a second run of the same length with HIP-event timers on the
-    library stream for per-kernel averages."""
+    library stream for per-kernel averages and the reference's phase split.
+    Energy drift is taken with the energy-shifted potential U(r) - U(rc)
+    (continuous at the cutoff) plus the kinetic energy."""
     sim = tm.Simulation.fcc(NC, precision=precision)
     sim.setup(half, RC, SKIN)
     sim.lennard_jones(half, RC)
     if warmup:
         sim.run_lj(warmup, REN, DT, half, RC, SKIN)
     sim.barrier()
-    th0 = sim.thermo(RC)
+    th0 = sim.thermo(RC, eshift=True)
     t0 = time.perf_counter()
     done = sim.run_lj(steps, REN, DT, half, RC, SKIN)
     sim.barrier()
     t1 = time.perf_counter()
     assert done == steps
-    th1 = sim.thermo(RC)
+    th1 = sim.thermo(RC, eshift=True)
     e0, e1 = th0["KE"] + th0["PE"], th1["KE"] + th1["PE"]
     out = {"seconds": t1 - t0, "atom_steps_per_s": sim.nlocal * steps / (t1 - t0),
            "energy_per_atom_start": e0 / sim.nlocal, "energy_per_atom_end": e1 / sim.nlocal,
-           "energy_drift_rel": (e1 - e0) / abs(e0), "T_end": th1["T"], "PE_end": th1["PE"], "P_end": th1["P"]}
+           "energy_drift_rel": (e1 - e0) / abs(e0), "energy_form": "KE + sum over in-cutoff pairs of U(r) - U(rc)",
+           "T_end": th1["T"], "P_end": th1["P"]}
     if profile:
         sim.profile(True)
         sim.profile_reset()
@@ -77,25 +104,33 @@ def run_variant(tm, precision, half, steps, warmup, profile=False):
         ms, cnt = sim.profile_get("force")
         out["force_ms_avg"] = ms / max(cnt, 1)
         out["force_launches"] = cnt
-        for k in ("integrate1", "integrate2", "ghost_sync", "mirror", "neighbor", "bin", "borders"):
+        per = {}
+        for k in ("force", "integrate1", "integrate2", "ghost_sync", "mirror", "neighbor", "bin", "borders"):
             kms, kc = sim.profile_get(k)
-            out[k + "_ms_avg"] = kms / max(kc, 1)
+            per[k] = kms
+            if k != "force":
+                out[k + "_ms_avg"] = kms / max(kc, 1)
+        out["phases_ms_per_step"] = {ph: sum(per[k] for k in ks) / steps for ph, ks in PHASES.items()}
         sim.profile(False)
     return sim, out
 
 
-def fp32_force_error(tm, sim):
-    """Force error of the FP32-mixed kernel against the FP64 kernel on the
-    same (thermalised) positions and neighbour list."""
+def reduced_force_error(tm, sim, precision):
+    """Per-atom relative force error of a reduced-precision kernel against
+    the FP64 kernel on the same (thermalised) positions and neighbour list:
+    |F_i - F64_i| / max(|F64_i|, floor) with the floor the RMS of |F64_i|,
+    as RMS and maximum over the atoms."""
     sim.set_precision("fp64")
     sim.lennard_jones(False, RC)
     f64 = sim.state()[3]
-    sim.set_precision("fp32mixed")
+    sim.set_precision(precision)
     sim.lennard_jones(False, RC)
     f32 = sim.state()[3]
-    rms = np.sqrt(np.mean(np.sum(f64 * f64, axis=1)))
-    d = f32 - f64
-    return float(np.sqrt(np.mean(np.sum(d * d, axis=1))) / rms), float(np.max(np.abs(d)) / rms)
+    sim.set_precision("fp32mixed")
+    fn = np.sqrt(np.sum(f64 * f64, axis=1))
+    floor = np.sqrt(np.mean(fn * fn))
+    d = np.sqrt(np.sum((f32 - f64) ** 2, axis=1)) / np.maximum(fn, floor)
+    return {"rms_rel": float(np.sqrt(np.mean(d * d))), "max_rel": float(np.max(d)), "floor": float(floor)}
 
 
 def cpu_model():
@@ -108,22 +143,36 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(sample_steps=20):
-    """FP64 oracle (test infrastructure, the checker) on the host cores: the
-    same 80^3 workload, `sample_steps` steps including one rebuild."""
+def cpu_baseline():
+    """FP64 oracle (test infrastructure, the checker, built with -O3
+    -fno-fast-math -ffp-contract=off -fopenmp) on the host cores, timed over
+    the region main.cpp:712-778 times (after the initial build):
+      - FCC 32^3 (131,072 atoms), 100 steps, 1 thread and `threads` threads;
+      - FCC 80^3 (2,048,000 atoms), 200 steps on `threads` threads, and a
+        20-step sample (one rebuild) on 1 thread, which stands for the
+        200-step run (the step cost is flat: one rebuild per 20 steps).
+    `value` is the 80^3 rate on `threads` threads (the GPU workload, FP64)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     threads = min(os.cpu_count() or 1, 16)
-    s = po.lj_benchmark(NC, threads=threads)
-    t0 = time.perf_counter()
-    s.run_lj(sample_steps, REN, DT)
-    t1 = time.perf_counter()
-    n = s.n
-    s.close()
-    return {"value": n * sample_steps / (t1 - t0), "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, FCC 80^3 ({n} atoms), {sample_steps} steps incl. 1 rebuild, "
-                      f"{threads} OpenMP threads (list build and pair force threaded), {t1 - t0:.1f} s",
-            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    runs = {}
+    for nc, steps, th in ((32, 100, 1), (32, 100, threads), (80, 200, threads), (80, 20, 1)):
+        s = po.lj_benchmark(nc, threads=th)
+        t0 = time.perf_counter()
+        s.run_lj(steps, REN, DT)
+        t1 = time.perf_counter()
+        n = s.n
+        s.close()
+        runs[f"{nc}cube_{steps}steps_{th}thread"] = {"atoms": n, "steps": steps, "threads": th, "seconds": t1 - t0,
+                                                       "atom_steps_per_s": n * steps / (t1 - t0)}
+        print(f"# cpu oracle {nc}^3 {steps} steps {th} threads: {n * steps / (t1 - t0):.4g} atom-steps/s",
+              file=sys.stderr, flush=True)
+    head = runs[f"80cube_200steps_{threads}thread"]
+    return {"value": head["atom_steps_per_s"], "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, FCC 80^3 ({head['atoms']} atoms), 200 steps incl. 10 rebuilds, full list, "
+                      f"{threads} OpenMP threads (list build and pair force threaded), {head['seconds']:.1f} s; "
+                      f"also 32^3 x 100 steps on 1 and {threads} threads and 80^3 x 20 steps on 1 thread (runs)",
+            "runs": runs, "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def c1_leg(tm, nc=32, steps=100, cpu=True):
@@ -148,22 +197,11 @@ def c1_leg(tm, nc=32, steps=100, cpu=True):
         import pyoracle as po
         threads = min(os.cpu_count() or 1, 16)
         s = po.lj_benchmark(nc, threads=threads)
-        c0 = time.perf_counter()
         s.run_lj(steps, REN, DT)
-        c1 = time.perf_counter()
         s.lj(False, RC)
         tc = s.thermo()
         s.close()
-        out.update({"cpu_seconds": c1 - c0, "cpu_atom_steps_per_s": n * steps / (c1 - c0), "cpu_threads": threads,
-                    "cpu_T": tc["T"], "cpu_PE": tc["PE"], "cpu_P": tc["P"],
-                    "gpu_over_cpu": (c1 - c0) / (t1 - t0)})
-        # one thread, on a 20-step sample of the same run
-        s1 = po.lj_benchmark(nc, threads=1)
-        d0 = time.perf_counter()
-        s1.run_lj(20, REN, DT)
-        d1 = time.perf_counter()
-        s1.close()
-        out.update({"cpu_1thread_atom_steps_per_s": n * 20 / (d1 - d0), "cpu_1thread_sample_steps": 20})
+        out.update({"cpu_T": tc["T"], "cpu_PE": tc["PE"], "cpu_P": tc["P"]})
     return out
 
 
@@ -184,7 +222,8 @@ def adjoint_leg(tm, precision, nc=16, steps=200, ck=20, fd=False):
     r = sim.traj_adjoint(steps, ck, DT, RC, SKIN, 1.0, 1.0, u_t, p_t)
     t3 = time.perf_counter()
     out = {"atoms": sim.nlocal, "steps": steps, "checkpoint_every": ck, "forward_s": t1 - t0,
-           "adjoint_s": t3 - t2, "adjoint_over_forward": (t3 - t2) / (t1 - t0), "loss": r["loss"],
+           "adjoint_s": t3 - t2, "adjoint_over_forward": (t3 - t2) / (t1 - t0),
+           "loss": r["loss"],
            "grad_eps": r["grad_eps"], "grad_sigma": r["grad_sigma"],
            "grad_x0_norm": float(np.linalg.norm(r["grad_x0"]))}
     if fd:
@@ -201,33 +240,61 @@ def adjoint_leg(tm, precision, nc=16, steps=200, ck=20, fd=False):
         out["fd_grad_eps"], out["fd_grad_sigma"] = fe, fs
         out["fd_rel_err_eps"] = abs(r["grad_eps"] - fe) / abs(fe)
         out["fd_rel_err_sigma"] = abs(r["grad_sigma"] - fs) / abs(fs)
+        if precision != "fp64":
+            # the same adjoint kernel on the FP64 forward trajectory: what is
+            # left of the error is the adjoint's own rounding
+            sim.set_adjoint_precision(precision)
+            r2 = sim.traj_adjoint(steps, ck, DT, RC, SKIN, 1.0, 1.0, u_t, p_t, grad_state=False)
+            sim.set_adjoint_precision(None)
+            out["fp64_forward_fd_rel_err_eps"] = abs(r2["grad_eps"] - fe) / abs(fe)
+            out["fp64_forward_fd_rel_err_sigma"] = abs(r2["grad_sigma"] - fs) / abs(fs)
     sim.close()
+    # a plain forward trajectory of the same length (no per-step energy and
+    # pressure): the step loop alone, from the same initial state (the
+    # library holds one simulation at a time, so after the one above closes)
+    sim2 = tm.Simulation.fcc(nc, precision=precision, lj_shift=True)
+    sim2.setup(False, RC, SKIN)
+    sim2.lennard_jones(False, RC)
+    sim2.run_lj(steps, ck, DT, False, RC, SKIN)  # warm-up
+    sim2.barrier()
+    p0 = time.perf_counter()
+    sim2.run_lj(steps, ck, DT, False, RC, SKIN)
+    sim2.barrier()
+    p1 = time.perf_counter()
+    sim2.close()
+    out.update({"plain_forward_s": p1 - p0, "adjoint_over_plain_forward": out["adjoint_s"] / (p1 - p0),
+                "forward_form": "forward_s: the loss trajectory (step loop plus per-step energy and pressure); "
+                                "plain_forward_s: the step loop alone"})
     return out
 
 
-def batched_leg(tm, steps=500, nrep=64, seed=2024, sequential=True):
+def batched_leg(tm, steps=500, nrep=64, seed=2024, sequential=True, autotune=True, graph=True):
     """64 independent LJ boxes (cells per side uniform 6..24, density
     U[0.70, 0.90], temperature U[0.7, 2.0]; counter-based generator, seed
     2024), FP32-mixed, cutoff 2.5, skin 0.3, dt 0.005, reneighbour 20:
     autotuned batched run (one hipGraph launch) against the same boxes run
-    one after another through the single-box path."""
+    one after another through the single-box path.  autotune=False keeps the
+    default configuration; graph=False launches the run's kernels one by one
+    (the same kernels and results) for tools that trace single launches."""
     boxes = tm.replica_boxes(seed=seed, nrep=nrep)
     atoms = int(sum(len(b[1]) for b in boxes))
     b = tm.Batch(boxes, cutoff=RC, skin=SKIN, dt=DT, reneigh=REN, precision="fp32mixed")
     t0 = time.perf_counter()
-    cfg = b.autotune(trial_steps=2 * REN)
+    cfg = b.autotune(trial_steps=2 * REN) if autotune else None
     t_tune = time.perf_counter() - t0
-    b.run(2 * REN)  # graph capture for the timed length happens below; this warms the kernels
+    if not graph:
+        b.set_graph(False)
+    b.run(2 * REN)  # warms the kernels
     b.reset()
-    b.run(steps)    # captures the `steps` graph
+    b.run(steps)    # with graphs: captures the `steps` graph
     b.reset()
     t0 = time.perf_counter()
     b.run(steps)
     t1 = time.perf_counter()
-    gpu_ms = None
     b.reset()
     gpu_ms = b.run_timed(steps)
-    out = {"replicas": nrep, "atoms": atoms, "steps": steps, "config": cfg, "autotune_s": t_tune,
+    out = {"replicas": nrep, "atoms": atoms, "steps": steps, "config": cfg, "autotuned": autotune,
+           "graph": graph, "autotune_s": t_tune,
            "batched_s": t1 - t0, "batched_atom_steps_per_s": atoms * steps / (t1 - t0),
            "batched_graph_ms": gpu_ms}
     b.close()
@@ -275,10 +342,20 @@ def dem_leg(tm, steps=1000, nc=64):
     ms, cnt = sim.profile_get("force")
     sim.profile(False)
     n = sim.nlocal
+    nall = n + sim.nghost
+    counts, _ = sim.neighbor_lists(with_lists=False)
+    entries = int(counts.astype(np.int64).sum())
+    # half-list form of the pair-force bytes: counts, 4-byte entries, FP64
+    # positions and velocities of locals and ghosts once, the force out and
+    # the reaction partials read once
+    nbytes = 4 * n + 4 * entries + 48 * nall + 24 * n + 24 * n
     ke = sim.thermo(1.0)["KE"]
     sim.close()
+    t = ms / max(cnt, 1)
     return {"atoms": n, "steps": steps, "seconds": t1 - t0, "atom_steps_per_s": n * steps / (t1 - t0),
-            "force_ms_avg": ms / max(cnt, 1), "kinetic_energy_end": ke}
+            "force_ms_avg": t, "kinetic_energy_end": ke, "half_list_entries": entries,
+            "roofline": {"bound": "hbm", "bytes_per_launch": nbytes, "achieved": nbytes / (t * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -328,16 +405,23 @@ def hbm_copy_gbs(nbytes=1 << 30, reps=5):
     return gbs
 
 
-def load_traffic():
-    """HBM bytes per launch of the force kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+def load_profile_figures():
+    """Per-launch figures of the force kernel from the committed rocprofv3
+    summaries: HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and the
+    FP64 and FP32 VALU flops of its counted instructions."""
     import re
-    p = os.path.join(ROOT, "profiles", "force_lj_fp32mixed_80cube_traffic.md")
-    if os.path.exists(p):
-        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(p).read(), re.M)
-        if m:
-            return int(m.group(1))
-    return None
+    out = {}
+    for name, keys in (("force_lj_fp32mixed_80cube_traffic.md", ("traffic_bytes_per_launch",)),
+                       ("force_lj_80cube_valu_flops.md", ("valu_f64_flops_per_launch", "valu_f32_flops_per_launch"))):
+        p = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(p):
+            continue
+        text = open(p).read()
+        for k in keys:
+            m = re.search(r"^" + k + r":\s*([0-9.eE+]+)", text, re.M)
+            if m:
+                out[k] = float(m.group(1))
+    return out
 
 
 def main():
@@ -350,6 +434,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-trajectory leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM granular-gas leg")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="batched leg with the default configuration, kernels launched one by one, 100 steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--copy-bw", action="store_true", help="only measure the device copy bandwidth and exit")
@@ -377,14 +463,15 @@ def main():
     sim32, r32 = run_variant(tm, "fp32mixed", False, args.steps, args.warmup, profile=True)
     variants["fp32mixed_full"] = r32
     print(f"# fp32mixed full: {r32}", file=sys.stderr, flush=True)
-    nbytes, entries = force_kernel_bytes(sim32, "fp32mixed")
-    err_rms, err_max = fp32_force_error(tm, sim32)
-    pairs_per_atom = sim32.thermo(RC)["pairs"] / 2.0 / sim32.nlocal  # unique in-cutoff pairs
+    model = force_kernel_model(sim32)
+    errors = {"fp32mixed": reduced_force_error(tm, sim32, "fp32mixed"),
+              "fp32kahan": reduced_force_error(tm, sim32, "fp32kahan")}
+    print(f"# force model {model}; errors vs FP64 {errors}", file=sys.stderr, flush=True)
     n = sim32.nlocal
     sim32.close()
     del sim32
     if not args.quick:
-        for prec, half in (("fp64", False), ("fp32mixed", True), ("fp64", True)):
+        for prec, half in (("fp64", False), ("fp32kahan", False), ("fp32mixed", True), ("fp64", True)):
             sim, r = run_variant(tm, prec, half, args.steps, args.warmup, profile=True)
             variants[f"{prec}_{'half' if half else 'full'}"] = r
             print(f"# {prec} half={half}: {r}", file=sys.stderr, flush=True)
@@ -403,7 +490,10 @@ def main():
         print(f"# c1 32^3 fp64: {c1}", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batch:
-        batched = batched_leg(tm, sequential=not args.quick)
+        if args.batch_profile:
+            batched = batched_leg(tm, steps=100, sequential=False, autotune=False, graph=False)
+        else:
+            batched = batched_leg(tm, sequential=not args.quick)
         print(f"# batched: {batched}", file=sys.stderr, flush=True)
     dem = None
     if not args.no_dem:
@@ -412,9 +502,19 @@ def main():
 
     value = r32["atom_steps_per_s"]
     t_force = r32["force_ms_avg"] * 1e-3
-    achieved = nbytes / t_force / 1e9
-    traffic = load_traffic()
+    achieved = model["bytes"] / t_force / 1e9
+    prof = load_profile_figures()
     copy_gbs = None if args.no_copy_bw else hbm_copy_gbs_child()
+    valu = {"alg_flops_per_launch": model["flops"], "alg_tflops": model["flops"] / t_force / 1e12,
+            "frac_fp64_peak": model["flops"] / t_force / 1e12 / FP64_PEAK_TFLOPS,
+            "frac_fp32_peak": model["flops"] / t_force / 1e12 / FP32_PEAK_TFLOPS,
+            "peaks_tflops": {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS}}
+    if "valu_f64_flops_per_launch" in prof:
+        f64 = prof["valu_f64_flops_per_launch"] / t_force / 1e12
+        f32 = prof["valu_f32_flops_per_launch"] / t_force / 1e12
+        valu.update({"counted_fp64_tflops": f64, "counted_fp32_tflops": f32,
+                     "counted_frac_fp64_peak": f64 / FP64_PEAK_TFLOPS, "counted_frac_fp32_peak": f32 / FP32_PEAK_TFLOPS,
+                     "counted_source": "profiles/force_lj_80cube_valu_flops.md (SQ_INSTS_VALU_* x 64 lanes, FMA x 2)"})
     line = {
         "metric": METRIC,
         "value": value,
@@ -426,19 +526,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair forces (compensated), f64 state",
+        "dtype": "f32 pair forces, f64 accumulation and state",
         "data": "synthetic: FCC lattice + Park-Miller velocities (initialize.h), T=1.44",
         "config": {"workload": "LJ FCC 80x80x80 cells, 2,048,000 atoms, rho 0.8442, rc 2.5, skin 0.3, "
                                "dt 0.005, reneighbour 20, full list, FP32-mixed",
-                   "atoms": n, "list_entries": entries},
+                   "atoms": n, "list_entries": model["list_entries"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_lj_full<FP32-mixed, kick fused>", "bytes_per_launch": nbytes,
-                     "avg_launch_ms": r32["force_ms_avg"],
-                     "copy_measured": copy_gbs, "frac_of_copy": achieved / copy_gbs if copy_gbs else None},
-        "fp32_vs_fp64_force_err": {"rms_rel": err_rms, "max_abs_over_rms": err_max},
-        "pair_interactions_per_s": value * pairs_per_atom,
-        "list_entries_per_s": value * entries / n,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("traffic_bytes_per_launch"),
+                     "kernel": "k_lj_full<FP32-mixed, kick and drift fused> (step-loop force pass)",
+                     "bytes_per_launch": model["bytes"],
+                     "bytes_model": "4 N + 4 entries + 24 N_all + 24 N (list counts, 4-byte entries, FP64 "
+                                    "positions once, FP64 force out)",
+                     "avg_launch_ms": r32["force_ms_avg"], "avg_launch_source": "HIP events on the library stream",
+                     "frac_2B_entries": model["bytes_2B_entries"] / t_force / 1e9 / HBM_PEAK_GBS,
+                     "frac_fused_step_model": model["bytes_fused_step_model"] / t_force / 1e9 / HBM_PEAK_GBS,
+                     "copy_measured": copy_gbs, "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
+                     "valu": valu},
+        "force_model": model,
+        "reduced_precision_force_err": errors,
+        "pair_interactions_per_s": value * model["in_cutoff_entries"] / 2.0 / n,
+        "list_entries_per_s": value * model["list_entries"] / n,
         "variants": variants,
     }
     if adjoint:

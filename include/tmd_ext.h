@@ -29,6 +29,12 @@ double tmd_kahan_sum(const float* x, int n);
  *     positions, distance test, velocities, wrap, ghosts and reductions),
  * 2 = as 1 but accumulated in FP32 with Kahan compensation (full lists). */
 void tmd_set_precision(int mode);
+/* Precision of the adjoint kernel alone (tmd_traj_adjoint's reverse pass):
+ * -1 (default) follows tmd_set_precision, 0 = FP64, 1 = FP32-mixed (its
+ * Hessian-vector product's pair scalars in FP32).  Setting it apart from the
+ * forward precision separates the adjoint's own rounding from that of the
+ * FP32-mixed forward trajectory. */
+void tmd_set_adjoint_precision(int mode);
 /* Half-list filter.  0 = local j < i plus every ghost (each periodic pair
  * once; default), 1 = the reference's literal i > j filter, which drops all
  * ghost pairs (core/neighborlist.impala:34). */
@@ -59,6 +65,11 @@ void tmd_set_tile_threads(int threads);
  * PE and the pair virial W = sum r.F over the current list (the reference
  * computes neither), P = (sum v^2 + W) / (3V), pairs = in-cutoff entries. */
 void tmd_thermo(double rc, double epsilon, double sigma, double* out6);
+/* As tmd_thermo, with PE taken over the energy-shifted potential U(r) - U(rc)
+ * for every in-cutoff pair (continuous at the cutoff; the plain-cutoff force
+ * is unchanged), for energy-drift bookkeeping.  Runs with the smooth cutoff
+ * (tmd_set_lj_shift) report their own shifted energy either way. */
+void tmd_thermo_eshift(double rc, double epsilon, double sigma, double* out6);
 
 /* ---- introspection for parity tests (host copies) --------------------------- */
 /* {nx, ny, nz, ncells, nlocal, nghost, list capacity} */

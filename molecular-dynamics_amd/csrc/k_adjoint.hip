@@ -88,24 +88,26 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_adjoint(const double4* _
             const double rx = pi.x - xj, ry = pi.y - yj, rz = pi.z - zj;
             const double s = rx * rx + ry * ry + rz * rz;
             if (!(s < a.rc2)) continue;
-            double f, df, u, dfds, duds;  // f(s), f'(s), u(s), d f/d sigma, d u/d sigma
+            // f(s), f'(s), u(s), d f/d sigma, d u/d sigma in FP64.  FP64 takes
+            // 1/s from the IEEE division (the forward kernel's rounding);
+            // FP32-mixed from the FP32 reciprocal refined by one FP64 Newton
+            // step, whose relative error is about the square of the FP32 one
+            // (~1e-14), at the cost of the FP32 path
+            double sr2;
             if (PREC == PREC_FP64) {
-                const double sr2 = 1.0 / s;
+                sr2 = 1.0 / s;
+            } else {
+                sr2 = (double)__builtin_amdgcn_rcpf((float)s);
+                sr2 = fma(sr2, fma(-s, sr2, 1.0), sr2);
+            }
+            double f, df, u, dfds, duds;
+            {
                 const double sr6 = sr2 * sr2 * sr2 * a.sigma6;
                 f = 48.0 * a.eps * sr6 * (sr6 - 0.5) * sr2;
                 df = 24.0 * a.eps * sr2 * sr2 * sr6 * (4.0 - 14.0 * sr6);
                 u = 4.0 * a.eps * sr6 * (sr6 - 1.0);
                 dfds = 144.0 * a.eps * inv_sig * sr2 * sr6 * (4.0 * sr6 - 1.0);
                 duds = 24.0 * a.eps * inv_sig * sr6 * (2.0 * sr6 - 1.0);
-            } else {
-                const float sr2 = __builtin_amdgcn_rcpf((float)s);
-                const float sr6 = sr2 * sr2 * sr2 * a.sigma6f;
-                const float e = a.epsf, is = (float)inv_sig;
-                f = 48.0f * e * sr6 * (sr6 - 0.5f) * sr2;
-                df = 24.0f * e * sr2 * sr2 * sr6 * (4.0f - 14.0f * sr6);
-                u = 4.0f * e * sr6 * (sr6 - 1.0f);
-                dfds = 144.0f * e * is * sr2 * sr6 * (4.0f * sr6 - 1.0f);
-                duds = 24.0f * e * is * sr6 * (2.0f * sr6 - 1.0f);
             }
             if (SHIFT) {
                 const double ds = s - a.rc2;
@@ -249,7 +251,9 @@ static void launch_lj_adjoint(Sim& s, double rc, double eps, double sigma, const
                        (const int*)s.tile_start.as<int>(), (const int*)s.nb_count.as<int>(),                            \
                        (const nb_t*)s.nb_list.as<nb_t>(), (const int*)s.g_owner.as<int>(),                             \
                        (const double4*)s.adj_fbar.as<double4>(), a, c, s.adj_xbar.as<double4>(), part)
-    if (s.precision == PREC_FP64) {
+    // the adjoint kernel's precision: the forward's unless set apart
+    const int prec = s.adj_precision >= 0 ? s.adj_precision : s.precision;
+    if (prec == PREC_FP64) {
         if (s.lj_shift) TMD_ADJ(PREC_FP64, 1); else TMD_ADJ(PREC_FP64, 0);
     } else {
         if (s.lj_shift) TMD_ADJ(PREC_FP32_MIXED, 1); else TMD_ADJ(PREC_FP32_MIXED, 0);

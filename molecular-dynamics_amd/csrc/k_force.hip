@@ -207,24 +207,33 @@ struct DEMArgs {
     TileGeom g;
 };
 
+// One contact term.  The caller decides contact (rsq < (2R)^2) in FP64 for
+// both precisions, so FP32-mixed forces switch on and off at exactly the
+// FP64 overlap: the dashpot makes the force jump by g_n v_n at first
+// contact, and a contact decided on the rounded FP32 rsq would add or drop
+// such a jump.  The overlap is passed as (2R)^2 - rsq, formed in FP64
+// before any rounding to R; FP32 turns it into delta = ((2R)^2 - r^2) /
+// (2R + r), since 2R - r in FP32 would lose a small overlap's leading
+// digits.
 template <typename R>
-__device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R rsq, R vx, R vy, R vz, R& Fx, R& Fy, R& Fz) {
-    Fx = Fy = Fz = R(0);
+__device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R rsq, R dd, R vx, R vy, R vz, R& Fx, R& Fy,
+                                         R& Fz) {
     const R d2 = (R)a.d2;
-    if (rsq < d2 * d2) {
-        const R r = sqrt(rsq);
-        const R delta = d2 - r;
-        const R nx = dx / r, ny = dy / r, nz = dz / r;
-        const R vn = vx * nx + vy * ny + vz * nz;
-        const R tx = vx - vn * nx, ty = vy - vn * ny, tz = vz - vn * nz;
-        const R vtl = sqrt(tx * tx + ty * ty + tz * tz);
-        const R fn = (R)a.k * delta - (R)a.gn * vn;
-        const R ft = fmin((R)a.gt * vtl, (R)a.mu * fabs(fn));
-        const R inv = vtl > R(0) ? ft / vtl : R(0);
-        Fx = fn * nx - inv * tx;
-        Fy = fn * ny - inv * ty;
-        Fz = fn * nz - inv * tz;
-    }
+    const R r = sqrt(rsq);
+    // FP64 keeps the reference's 2R - r
+    R delta;
+    if constexpr (sizeof(R) == sizeof(double)) delta = d2 - r;
+    else delta = dd / (d2 + r);
+    const R nx = dx / r, ny = dy / r, nz = dz / r;
+    const R vn = vx * nx + vy * ny + vz * nz;
+    const R tx = vx - vn * nx, ty = vy - vn * ny, tz = vz - vn * nz;
+    const R vtl = sqrt(tx * tx + ty * ty + tz * tz);
+    const R fn = (R)a.k * delta - (R)a.gn * vn;
+    const R ft = fmin((R)a.gt * vtl, (R)a.mu * fabs(fn));
+    const R inv = vtl > R(0) ? ft / vtl : R(0);
+    Fx = fn * nx - inv * tx;
+    Fy = fn * ny - inv * ty;
+    Fz = fn * nz - inv * tz;
 }
 
 // DEM forces by gather.  PAIR2 (half lists): the atom also gathers its
@@ -278,14 +287,16 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS,
                     }
                     const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                     const double rsq = dx * dx + dy * dy + dz * dz;
-                    if (rsq < a.rc2) {
+                    // a contact also needs rsq < (2R)^2, decided here in FP64
+                    const double dd = a.d2 * a.d2 - rsq;
+                    if (rsq < a.rc2 && dd > 0.0) {
                         const double4 vj = vel[j];
                         double Fx, Fy, Fz;
                         if (PREC == PREC_FP64) {
-                            dem_pair<double>(a, dx, dy, dz, rsq, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, Fx, Fy, Fz);
+                            dem_pair<double>(a, dx, dy, dz, rsq, dd, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, Fx, Fy, Fz);
                         } else {
                             float gx, gy, gz;
-                            dem_pair<float>(a, (float)dx, (float)dy, (float)dz, (float)rsq, (float)(vi.x - vj.x),
+                            dem_pair<float>(a, (float)dx, (float)dy, (float)dz, (float)rsq, (float)dd, (float)(vi.x - vj.x),
                                             (float)(vi.y - vj.y), (float)(vi.z - vj.z), gx, gy, gz);
                             Fx = gx; Fy = gy; Fz = gz;
                         }
@@ -416,8 +427,9 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
     TMD_HIP(hipGetLastError());
 }
 
-void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part) {
+void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part, bool eshift) {
     LJArgs a = make_lj_args(s, rc, eps, sigma);
+    if (eshift && !s.lj_shift) a.eshift = a.uc;
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     if (!s.mirror_fresh) refresh_mirror(s);
     if (s.lj_shift)
@@ -427,14 +439,16 @@ void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part)
     TMD_HIP(hipGetLastError());
 }
 
-void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs) {
+void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs, bool eshift,
+                      double* buf) {
     *pe = 0.0; *vir = 0.0; *pairs = 0;
     if (has_error() || !s.nb_valid || s.n == 0) return;
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
-    s.red.reserve(sizeof(double) * (3 * (size_t)nb + 8), false, s.stream);
-    double* part = s.red.as<double>();
+    if (!buf) s.red.reserve(sizeof(double) * (3 * (size_t)nb + 8), false, s.stream);
+    if (has_error()) return;
+    double* part = buf ? buf : s.red.as<double>();
     double* out = part + 3 * (size_t)nb;
-    launch_lj_energy(s, rc, eps, sigma, part);
+    launch_lj_energy(s, rc, eps, sigma, part, eshift);
     reduce_partials(s, part, nb, 3, out);
     double host[3] = {0, 0, 0};
     TMD_HIP(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, s.stream));

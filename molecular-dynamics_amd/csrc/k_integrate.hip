@@ -116,11 +116,12 @@ int launch_kinetic(Sim& s, double* part) {
     return nb;
 }
 
-double kinetic_sum(Sim& s) {
+double kinetic_sum(Sim& s, double* buf) {
     if (has_error() || s.n == 0) return 0.0;
     const int nb = div_up(s.n, 256);
-    s.red.reserve(sizeof(double) * ((size_t)nb + 8), false, s.stream);
-    double* part = s.red.as<double>();
+    if (!buf) s.red.reserve(sizeof(double) * ((size_t)nb + 8), false, s.stream);
+    if (has_error()) return 0.0;
+    double* part = buf ? buf : s.red.as<double>();
     double* out = part + nb;
     launch_kinetic(s, part);
     reduce_partials(s, part, nb, 1, out);

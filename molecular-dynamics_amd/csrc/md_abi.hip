@@ -659,6 +659,7 @@ void tmd_set_ghost_mode(int mode) { sim().ghost_mode = mode ? GHOST_REF_LAG : GH
 void tmd_set_sync(int on) { sim().sync_calls = on ? 1 : 0; }
 void tmd_set_dem_radius(double r) { sim().dem_radius = r; }
 void tmd_set_lj_shift(int on) { sim().lj_shift = on ? 1 : 0; }
+void tmd_set_adjoint_precision(int mode) { sim().adj_precision = mode < 0 ? -1 : (mode ? PREC_FP32_MIXED : PREC_FP64); }
 
 void tmd_set_tile_threads(int threads) {
     Sim& s = sim();
@@ -679,7 +680,7 @@ void tmd_set_tile(int tx, int ty, int tz) {
     s.nb_valid = 0;
 }
 
-void tmd_thermo(double rc, double eps, double sigma, double* out) {
+static void thermo(double rc, double eps, double sigma, double* out, bool eshift) {
     Sim& s = sim();
     for (int k = 0; k < 6; ++k) out[k] = 0.0;
     if (!ready(s)) return;
@@ -688,7 +689,7 @@ void tmd_thermo(double rc, double eps, double sigma, double* out) {
     const double sum_mv2 = kinetic_sum(s);
     double pe = 0.0, vir = 0.0;
     long long pairs = 0;
-    lj_energy_virial(s, rc, eps, sigma, &pe, &vir, &pairs);
+    lj_energy_virial(s, rc, eps, sigma, &pe, &vir, &pairs, eshift);
     s.timer.end(K_THERMO, s.stream, ev);
     const double T = sum_mv2 * (1.0 / (double)(s.n * 3 - 3));
     const double vol = s.g.len[0] * s.g.len[1] * s.g.len[2];
@@ -702,6 +703,10 @@ void tmd_thermo(double rc, double eps, double sigma, double* out) {
     out[4] = (sum_mv2 + vir) / (3.0 * vol);
     out[5] = (double)pairs;
 }
+
+void tmd_thermo(double rc, double eps, double sigma, double* out) { thermo(rc, eps, sigma, out, false); }
+
+void tmd_thermo_eshift(double rc, double eps, double sigma, double* out) { thermo(rc, eps, sigma, out, true); }
 
 void tmd_get_grid_info(int* out) {
     Sim& s = sim();

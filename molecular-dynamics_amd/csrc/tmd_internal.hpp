@@ -107,6 +107,7 @@ struct Sim {
     int lds_limit = 65536;  // LDS bytes per block (device attribute)
 
     int precision = PREC_FP64;
+    int adj_precision = -1;  // adjoint kernel precision (-1: the forward's)
     int half_mode = HALF_GHOST_FULL;
     int ghost_mode = GHOST_FRESH;
     int sync_calls = 1;
@@ -208,11 +209,15 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
 // step's first kick and drift
 void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double kick_dt = 0.0,
                  bool advance = false);
-void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs);
+// eshift: subtract u(rc) from each in-cutoff pair's energy (plain-cutoff runs)
+// buf: 3 * tiles + 8 doubles of device memory for the partial sums (nullptr:
+// the simulation's own reduction buffer)
+void lj_energy_virial(Sim& s, double rc, double eps, double sigma, double* pe, double* vir, long long* pairs,
+                      bool eshift = false, double* buf = nullptr);
 
 void reduce_partials(Sim& s, const double* partials, int nblocks, int width, double* out);
 // per-tile (PE, W, pairs) partials of the current full/half list into part[3 * ntiles]
-void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part);
+void launch_lj_energy(Sim& s, double rc, double eps, double sigma, double* part, bool eshift = false);
 
 // integration
 void initial_integration(Sim& s, double dt);
@@ -231,7 +236,9 @@ __host__ __device__ inline void pbc_wrap(double4& p, const PbcArgs& a) {
     q = p.y; if (p.y < a.lo[1]) q += a.len[1]; if (p.y > a.hi[1]) q -= a.len[1]; p.y = q;
     q = p.z; if (p.z < a.lo[2]) q += a.len[2]; if (p.z > a.hi[2]) q -= a.len[2]; p.z = q;
 }
-double kinetic_sum(Sim& s);  // sum m v.v over locals (deterministic)
+// sum m v.v over locals (deterministic); buf: ceil(n / 256) + 8 doubles of
+// device memory for the partial sums (nullptr: the simulation's own buffer)
+double kinetic_sum(Sim& s, double* buf = nullptr);
 int launch_kinetic(Sim& s, double* part);  // per-block partials of sum v.v; returns the block count
 
 // trajectory loss (Ubar, Pbar) and its adjoint (k_adjoint.hip)

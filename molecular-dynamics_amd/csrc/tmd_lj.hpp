@@ -7,6 +7,9 @@
 //   f_s(s) = f(s) - f(sc) - (s - sc) f'(sc)
 //   u_s(s) = u(s) - u(sc) + (s - sc) f(sc)/2 + (s - sc)^2 f'(sc)/4
 // so force and force derivative vanish at the cutoff (u_s' = -f_s / 2).
+// For energy bookkeeping of plain-cutoff runs, eshift = u(sc) can be
+// subtracted from each in-cutoff pair's energy (the energy-shifted
+// potential, continuous at the cutoff; the force is unchanged).
 #pragma once
 
 #include "tmd_internal.hpp"
@@ -15,6 +18,7 @@ namespace tmd {
 
 struct LJArgs {
     double rc2, rc, eps, sigma, sigma6, fc, dfc, uc;
+    double eshift;  // energy only: subtracted from each in-cutoff pair's U (0: none)
     float epsf, sigma6f, fcf, dfcf, rc2f;
     int cap, nwork, n, wcap;
     TileGeom g;
@@ -33,6 +37,7 @@ inline LJArgs make_lj_args(const Sim& s, double rc, double eps, double sigma) {
     a.fc = 48.0 * sr6 * (sr6 - 0.5) * sr2 * eps;
     a.dfc = 24.0 * eps * sr2 * sr2 * sr6 * (4.0 - 14.0 * sr6);
     a.uc = 4.0 * eps * sr6 * (sr6 - 1.0);
+    a.eshift = 0.0;
     a.epsf = (float)eps;
     a.sigma6f = (float)a.sigma6;
     a.fcf = (float)a.fc;

@@ -225,6 +225,8 @@ __device__ __forceinline__ void lj_energy_atom(const double4 pi, int cnt, const 
                     const double ds = rsq - a.rc2;
                     f = f - a.fc - ds * a.dfc;
                     uu = uu - a.uc + ds * (0.5 * a.fc) + ds * ds * (0.25 * a.dfc);
+                } else if (a.eshift != 0.0) {
+                    uu = uu - a.eshift;
                 }
                 const double wt = half ? (j < nloc_end ? 1.0 : 0.5) : 0.5;
                 pe += wt * uu;

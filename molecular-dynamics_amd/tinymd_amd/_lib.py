@@ -74,6 +74,7 @@ SIGNATURES = {
     "tmd_device_info": (_i, [ctypes.c_char_p, _i]),
     "tmd_kahan_sum": (_d, [ctypes.POINTER(ctypes.c_float), _i]),
     "tmd_set_precision": (None, [_i]),
+    "tmd_set_adjoint_precision": (None, [_i]),
     "tmd_set_half_mode": (None, [_i]),
     "tmd_set_ghost_mode": (None, [_i]),
     "tmd_set_sync": (None, [_i]),
@@ -82,6 +83,7 @@ SIGNATURES = {
     "tmd_set_tile": (None, [_i, _i, _i]),
     "tmd_set_tile_threads": (None, [_i]),
     "tmd_thermo": (None, [_d, _d, _d, _dp]),
+    "tmd_thermo_eshift": (None, [_d, _d, _d, _dp]),
     "tmd_get_grid_info": (None, [_ip]),
     "tmd_get_ghost_table": (_i, [_ip, _ip, _ip, _ip]),
     "tmd_get_cells": (_i, [_ip, _ip, _ip]),

@@ -84,6 +84,7 @@ class Simulation:
                   dem_radius=None, sync=True):
         L = self.L
         L.tmd_set_precision(PRECISION[precision])
+        L.tmd_set_adjoint_precision(-1)
         L.tmd_set_half_mode(HALF_MODE[half_mode])
         L.tmd_set_ghost_mode(GHOST_MODE[ghost_mode])
         L.tmd_set_lj_shift(int(bool(lj_shift)))
@@ -92,6 +93,10 @@ class Simulation:
 
     def set_precision(self, precision):
         self.L.tmd_set_precision(PRECISION[precision])
+
+    def set_adjoint_precision(self, precision=None):
+        """Precision of the adjoint kernel alone; None follows the forward."""
+        self.L.tmd_set_adjoint_precision(-1 if precision is None else PRECISION[precision])
 
     def set_tile(self, tx=0, ty=None, tz=None, threads=None):
         """Cells per tile (0: automatic) and threads per tile block (0:
@@ -276,9 +281,11 @@ class Simulation:
         check()
         return counts, lst
 
-    def thermo(self, cutoff=2.5, eps=1.0, sigma=1.0):
+    def thermo(self, cutoff=2.5, eps=1.0, sigma=1.0, eshift=False):
+        """T, KE, PE, virial, P, pairs; eshift: PE of the energy-shifted
+        potential U(r) - U(rc) (for energy-drift bookkeeping)."""
         out = np.zeros(6)
-        self.L.tmd_thermo(cutoff, eps, sigma, _dptr(out))
+        (self.L.tmd_thermo_eshift if eshift else self.L.tmd_thermo)(cutoff, eps, sigma, _dptr(out))
         check()
         return dict(zip(["T", "KE", "PE", "virial", "P", "pairs"], out.tolist()))
 

@@ -118,3 +118,38 @@ def test_adjoint_checkpoint_interval_invariance(tm):
     rg = gpu_sim(tm, system).traj_adjoint(30, 1000, DT, RC, SKIN, 1.0, 1.0, -4.0, 1.0)
     assert rg["grad_sigma"] == pytest.approx(ro["grad_sigma"], rel=1e-9)
     assert np.max(np.abs(rg["grad_v0"] - ro["grad_v0"])) <= 1e-9 * np.max(np.abs(ro["grad_v0"]))
+
+
+def test_adjoint_16cube_central_differences_by_precision(tm):
+    """Differentiable LJ at full size (FCC 16^3, 16,384 atoms, 200 steps,
+    checkpoints every 20, targets from the same state at eps = 1.05,
+    sigma = 0.98): parameter gradients against central differences (h = 1e-6)
+    of the FP64 loss.  The FP64 adjoint and the FP32-mixed adjoint kernel
+    run on the FP64 forward trajectory are within 1e-6 relative; with the
+    FP32-mixed forward trajectory as well, the gradient belongs to that
+    slightly different trajectory, and its distance from the FP64 finite
+    differences is reported and bounded at 1e-5."""
+    announce("adjoint 16^3, 200 steps, ck 20: gradients vs central differences by precision")
+    steps, ck = 200, 20
+    sim = tm.Simulation.fcc(16, precision="fp64", lj_shift=True)
+    u_t, p_t = sim.traj_loss(steps, ck, DT, RC, SKIN, 1.05, 0.98)
+
+    def loss(e, s):
+        u, p = sim.traj_loss(steps, ck, DT, RC, SKIN, e, s)
+        return (u - u_t) ** 2 + (p - p_t) ** 2
+    fe = (loss(1 + H, 1.0) - loss(1 - H, 1.0)) / (2 * H)
+    fs = (loss(1.0, 1 + H) - loss(1.0, 1 - H)) / (2 * H)
+    res = {}
+    for fwd, adj in (("fp64", "fp64"), ("fp64", "fp32mixed"), ("fp32mixed", "fp32mixed")):
+        sim.set_precision(fwd)
+        sim.set_adjoint_precision(adj)
+        r = sim.traj_adjoint(steps, ck, DT, RC, SKIN, 1.0, 1.0, u_t, p_t, grad_state=False)
+        res[(fwd, adj)] = (abs(r["grad_eps"] - fe) / abs(fe), abs(r["grad_sigma"] - fs) / abs(fs))
+        announce(f"  forward {fwd}, adjoint {adj}: relative error vs FD eps {res[(fwd, adj)][0]:.3e} "
+                 f"sigma {res[(fwd, adj)][1]:.3e}")
+    sim.set_adjoint_precision(None)
+    sim.set_precision("fp64")
+    sim.close()
+    for key in (("fp64", "fp64"), ("fp64", "fp32mixed")):
+        assert max(res[key]) <= 1e-6, (key, res[key])
+    assert max(res[("fp32mixed", "fp32mixed")]) <= 1e-5, res

@@ -162,6 +162,30 @@ def test_thermo_matches_oracle(tm, half):
     assert tg["pairs"] == to["pairs"]
 
 
+def test_thermo_energy_shift(tm):
+    """PE of the energy-shifted potential U(r) - U(rc): with a full list every
+    in-cutoff pair is listed twice at weight 1/2, so it is the oracle's PE
+    minus U(rc) times half the in-cutoff entries; the half list (each pair
+    once, ghost pairs at 1/2 from both owners) gives the same physical sum."""
+    announce("thermo with the energy-shifted potential, full and half lists")
+    aabb, m, x, v = perturbed_fcc(8, amp=0.1, seed=7)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc, False)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    to = orc.thermo()
+    ts = sim.thermo(RC, eshift=True)
+    urc = 4.0 * (RC ** -12 - RC ** -6)
+    expect = to["PE"] - urc * to["pairs"] / 2
+    assert abs(ts["PE"] - expect) <= 1e-12 * abs(expect), (ts["PE"], expect)
+    assert ts["pairs"] == to["pairs"] and ts["virial"] == pytest.approx(to["virial"], rel=1e-12)
+    sim.setup(True, RC, SKIN)
+    th = sim.thermo(RC, eshift=True)
+    assert abs(th["PE"] - expect) <= 1e-12 * abs(expect), (th["PE"], expect)
+    sim.close()
+    orc.close()
+
+
 def test_trajectory_bitwise_abi_and_runner(tm):
     announce("trajectory 6^3, 60 steps, ABI loop and in-library loop vs oracle")
     aabb, m, x, v = perturbed_fcc(6, amp=0.0, seed=0)
