@@ -355,6 +355,7 @@ def dem_leg(tm, steps=1000, nc=64):
     return {"atoms": n, "steps": steps, "seconds": t1 - t0, "atom_steps_per_s": n * steps / (t1 - t0),
             "force_ms_avg": t, "kinetic_energy_end": ke, "half_list_entries": entries,
             "roofline": {"bound": "hbm", "bytes_per_launch": nbytes, "achieved": nbytes / (t * 1e-3) / 1e9,
+                         "time": "the force phase by HIP events: the pair pass and the end-of-step kick pass",
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
@@ -407,12 +408,14 @@ def hbm_copy_gbs(nbytes=1 << 30, reps=5):
 
 def load_profile_figures():
     """Per-launch figures of the force kernel from the committed rocprofv3
-    summaries: HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and the
-    FP64 and FP32 VALU flops of its counted instructions."""
+    summaries: HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated), the
+    FP64 and FP32 VALU flops of its counted instructions, and its average
+    duration in the kernel trace."""
     import re
     out = {}
     for name, keys in (("force_lj_fp32mixed_80cube_traffic.md", ("traffic_bytes_per_launch",)),
-                       ("force_lj_80cube_valu_flops.md", ("valu_f64_flops_per_launch", "valu_f32_flops_per_launch"))):
+                       ("force_lj_80cube_valu_flops.md", ("valu_f64_flops_per_launch", "valu_f32_flops_per_launch")),
+                       ("lj_80cube_fp32mixed_full_kernel_stats.md", ("trace_avg_us",))):
         p = os.path.join(ROOT, "profiles", name)
         if not os.path.exists(p):
             continue
@@ -538,6 +541,9 @@ def main():
                      "bytes_model": "4 N + 4 entries + 24 N_all + 24 N (list counts, 4-byte entries, FP64 "
                                     "positions once, FP64 force out)",
                      "avg_launch_ms": r32["force_ms_avg"], "avg_launch_source": "HIP events on the library stream",
+                     "trace_avg_ms": prof["trace_avg_us"] * 1e-3 if "trace_avg_us" in prof else None,
+                     "frac_at_trace_avg": (model["bytes"] / (prof["trace_avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
+                                           if "trace_avg_us" in prof else None),
                      "frac_2B_entries": model["bytes_2B_entries"] / t_force / 1e9 / HBM_PEAK_GBS,
                      "frac_fused_step_model": model["bytes_fused_step_model"] / t_force / 1e9 / HBM_PEAK_GBS,
                      "copy_measured": copy_gbs, "frac_of_copy": achieved / copy_gbs if copy_gbs else None,

@@ -71,6 +71,23 @@ void tmd_thermo(double rc, double epsilon, double sigma, double* out6);
  * (tmd_set_lj_shift) report their own shifted energy either way. */
 void tmd_thermo_eshift(double rc, double epsilon, double sigma, double* out6);
 
+/* ---- state I/O -------------------------------------------------------------
+ * VTK: the reference's legacy particle file (vtk.h:6-91, written by main.cpp's
+ * vtk_write_*_data, main.cpp:67-98, from md_write_grid_*_data_to_arrays) for
+ * what = 0 (locals), 1 (ghosts) or 2 (the grid AABB's 8 corners), from the
+ * host copy (after md_copy_data_from_accelerator, as main.cpp does).
+ * Returns the number of points.
+ * Restart (no counterpart in the reference): tmd_save_state writes the
+ * locals' mass, position, velocity and force (FP64), the geometry and the
+ * modes to a binary file with a checksum; tmd_load_state reads one into a
+ * fresh simulation (md_initialize_grid + md_copy_data_to_accelerator, with
+ * the saved force).  Ghosts, cells and lists are rebuilt by the next
+ * set-up, as after md_initialize_grid.  Both return the number of atoms
+ * (0 and an error on failure). */
+int tmd_write_vtk(const char* path, int what);
+int tmd_save_state(const char* path);
+int tmd_load_state(const char* path);
+
 /* ---- introspection for parity tests (host copies) --------------------------- */
 /* {nx, ny, nz, ncells, nlocal, nghost, list capacity} */
 void tmd_get_grid_info(int* out7);

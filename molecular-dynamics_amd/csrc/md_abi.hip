@@ -211,7 +211,10 @@ static void set_geometry(Sim& s, const double* aabb, const double* rank_aabb, do
 }
 
 void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell_spacing) {
-    for (int d = 0; d < 6; ++d) G.world[d] = aabb[d];
+    for (int d = 0; d < 6; ++d) {
+        G.world[d] = aabb[d];
+        G.rank[d] = rank_aabb[d];
+    }
     for (int d = 0; d < 3; ++d) {
         G.aabb[2 * d] = rank_aabb[2 * d] - cell_spacing;
         G.aabb[2 * d + 1] = rank_aabb[2 * d + 1] + cell_spacing;
@@ -415,6 +418,169 @@ int md_write_grid_aabb_data_to_arrays(double* masses, struct Vector3D* positions
                 forces[idx] = {0.0, 0.0, 0.0};
             }
     return s.n;
+}
+
+// ---------------------------------------------------------------------------
+// State I/O beyond the reference's array exports: the reference's legacy VTK
+// particle file (vtk.h:6-91 through main.cpp:67-98) written by the library,
+// and a binary restart file.
+
+static bool vtk_file(const char* path, const std::vector<double>& m, const std::vector<Vector3D>& x,
+                     const std::vector<Vector3D>& v, const std::vector<Vector3D>& f) {
+    FILE* fp = fopen(path, "w");
+    if (!fp) return false;
+    const size_t n = x.size();
+    // the layout and number format of vtk.h (std::fixed: six decimals)
+    fprintf(fp, "# vtk DataFile Version 2.0\nParticle data\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+    fprintf(fp, "POINTS %zu double\n", n);
+    for (const Vector3D& p : x) fprintf(fp, "%f %f %f\n", p.x, p.y, p.z);
+    fprintf(fp, "\n\nCELLS %zu %zu\n", n, 2 * n);
+    for (size_t i = 0; i < n; ++i) fprintf(fp, "1 %zu\n", i);
+    fprintf(fp, "\n\nCELL_TYPES %zu\n", n);
+    for (size_t i = 0; i < n; ++i) fprintf(fp, "1\n");
+    fprintf(fp, "\n\nPOINT_DATA %zu\n", m.size());
+    fprintf(fp, "SCALARS mass double\nLOOKUP_TABLE default\n");
+    for (double q : m) fprintf(fp, "%f\n", q);
+    fprintf(fp, "\n");
+    const char* names[6] = {"velocity_x", "velocity_y", "velocity_z", "force_x", "force_y", "force_z"};
+    for (int c = 0; c < 6; ++c) {
+        const std::vector<Vector3D>& a = c < 3 ? v : f;
+        fprintf(fp, "SCALARS %s double\nLOOKUP_TABLE default\n", names[c]);
+        for (const Vector3D& q : a) fprintf(fp, "%f\n", c % 3 == 0 ? q.x : (c % 3 == 1 ? q.y : q.z));
+        fprintf(fp, "\n");
+    }
+    return fclose(fp) == 0;
+}
+
+int tmd_write_vtk(const char* path, int what) {
+    Sim& s = sim();
+    if (has_error()) return 0;
+    if (what < 0 || what > 2) {
+        set_error(__FILE__, __LINE__, "tmd_write_vtk: what must be 0 (locals), 1 (ghosts) or 2 (grid AABB)");
+        return 0;
+    }
+    const int n = what == 0 ? s.n : (what == 1 ? s.ng : 8);
+    std::vector<double> m(n);
+    std::vector<Vector3D> x(n), v(n), f(n);
+    if (what == 0) md_write_grid_data_to_arrays(m.data(), x.data(), v.data(), f.data());
+    else if (what == 1) md_write_grid_ghost_data_to_arrays(m.data(), x.data(), v.data(), f.data());
+    else md_write_grid_aabb_data_to_arrays(m.data(), x.data(), v.data(), f.data());
+    if (!vtk_file(path, m, x, v, f)) {
+        set_error(__FILE__, __LINE__, "tmd_write_vtk: cannot write the file");
+        return 0;
+    }
+    return n;
+}
+
+// Restart file: a fixed header, then the locals' mass, position, velocity
+// and force in FP64 (the arrays of md_initialize_grid plus the force that
+// the next step's first kick uses; the (x, v) pair is what the adjoint
+// checkpoints hold), then an FNV-1a checksum of everything before it.
+// Ghosts, cells and lists are derived state: a restored run rebuilds them,
+// as a run started from md_initialize_grid does.
+struct RestartHeader {
+    char magic[8];  // "TMDRST1" and a terminating zero
+    int version, n, nb_cap, precision, half_mode, ghost_mode, lj_shift, pad;
+    double world[6], rank[6], spacing, dem_radius;
+};
+
+static unsigned long long fnv1a(const void* p, size_t len, unsigned long long h) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < len; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+int tmd_save_state(const char* path) {
+    Sim& s = sim();
+    if (!ready(s)) return 0;
+    if (s.pos_stale || s.ghost_stale || s.ghost_pos_stale) sync_pos_from_mirror(s);
+    download(s);
+    if (has_error()) return 0;
+    RestartHeader h{};
+    std::memcpy(h.magic, "TMDRST1", 8);
+    h.version = 1;
+    h.n = s.n;
+    h.nb_cap = s.nb_cap;
+    h.precision = s.precision;
+    h.half_mode = s.half_mode;
+    h.ghost_mode = s.ghost_mode;
+    h.lj_shift = s.lj_shift;
+    for (int d = 0; d < 6; ++d) {
+        h.world[d] = s.g.world[d];
+        h.rank[d] = s.g.rank[d];
+    }
+    h.spacing = s.g.spacing;
+    h.dem_radius = s.dem_radius;
+    FILE* fp = fopen(path, "wb");
+    if (!fp) {
+        set_error(__FILE__, __LINE__, "tmd_save_state: cannot open the file");
+        return 0;
+    }
+    unsigned long long sum = 1469598103934665603ull;
+    const size_t n = (size_t)s.n;
+    bool ok = fwrite(&h, sizeof h, 1, fp) == 1;
+    sum = fnv1a(&h, sizeof h, sum);
+    const void* parts[4] = {s.h_mass.data(), s.h_pos.data(), s.h_vel.data(), s.h_force.data()};
+    const size_t lens[4] = {n, 3 * n, 3 * n, 3 * n};
+    for (int k = 0; k < 4 && ok; ++k) {
+        ok = fwrite(parts[k], sizeof(double), lens[k], fp) == lens[k];
+        sum = fnv1a(parts[k], sizeof(double) * lens[k], sum);
+    }
+    ok = ok && fwrite(&sum, sizeof sum, 1, fp) == 1;
+    ok = (fclose(fp) == 0) && ok;
+    if (!ok) {
+        set_error(__FILE__, __LINE__, "tmd_save_state: write failed");
+        return 0;
+    }
+    return s.n;
+}
+
+int tmd_load_state(const char* path) {
+    Sim& s = sim();
+    if (has_error()) return 0;
+    FILE* fp = fopen(path, "rb");
+    if (!fp) {
+        set_error(__FILE__, __LINE__, "tmd_load_state: cannot open the file");
+        return 0;
+    }
+    RestartHeader h{};
+    bool ok = fread(&h, sizeof h, 1, fp) == 1 && std::memcmp(h.magic, "TMDRST1", 8) == 0 && h.version == 1 &&
+              h.n >= 0 && h.n < (1 << 30);
+    std::vector<double> m, x, v, f;
+    unsigned long long sum = 1469598103934665603ull, stored = 0;
+    if (ok) {
+        const size_t n = (size_t)h.n;
+        m.resize(n);
+        x.resize(3 * n);
+        v.resize(3 * n);
+        f.resize(3 * n);
+        sum = fnv1a(&h, sizeof h, sum);
+        std::vector<double>* parts[4] = {&m, &x, &v, &f};
+        for (int k = 0; k < 4 && ok; ++k) {
+            ok = fread(parts[k]->data(), sizeof(double), parts[k]->size(), fp) == parts[k]->size();
+            sum = fnv1a(parts[k]->data(), sizeof(double) * parts[k]->size(), sum);
+        }
+        ok = ok && fread(&stored, sizeof stored, 1, fp) == 1 && stored == sum;
+    }
+    fclose(fp);
+    if (!ok) {
+        set_error(__FILE__, __LINE__, "tmd_load_state: not a restart file of this version, truncated or corrupted");
+        return 0;
+    }
+    md_initialize_grid(m.data(), reinterpret_cast<const Vector3D*>(x.data()), reinterpret_cast<const Vector3D*>(v.data()), h.n,
+                       h.world, h.rank, h.spacing, 0, h.nb_cap);
+    if (has_error()) return 0;
+    s.h_force = f;
+    s.precision = h.precision;
+    s.half_mode = h.half_mode;
+    s.ghost_mode = h.ghost_mode;
+    s.lj_shift = h.lj_shift;
+    s.dem_radius = h.dem_radius;
+    upload(s);
+    return has_error() ? 0 : h.n;
 }
 
 void md_distribute_particles() {

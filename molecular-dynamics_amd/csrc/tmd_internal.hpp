@@ -46,6 +46,7 @@ struct Geom {
     double aabb[6];   // rank AABB extended by one cell spacing
     double len[3];    // periodic lengths
     double spacing;   // cell spacing
+    double rank[6];   // rank AABB as given (aabb is it extended by one spacing)
     int nx, ny, nz, ncells;
 };
 

@@ -84,6 +84,9 @@ SIGNATURES = {
     "tmd_set_tile_threads": (None, [_i]),
     "tmd_thermo": (None, [_d, _d, _d, _dp]),
     "tmd_thermo_eshift": (None, [_d, _d, _d, _dp]),
+    "tmd_write_vtk": (_i, [ctypes.c_char_p, _i]),
+    "tmd_save_state": (_i, [ctypes.c_char_p]),
+    "tmd_load_state": (_i, [ctypes.c_char_p]),
     "tmd_get_grid_info": (None, [_ip]),
     "tmd_get_ghost_table": (_i, [_ip, _ip, _ip, _ip]),
     "tmd_get_cells": (_i, [_ip, _ip, _ip]),

@@ -4,6 +4,7 @@ simulation per process (core/runtime.impala:1-5), so a Simulation object is
 a handle to that state, and creating a new one replaces the previous one.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -107,6 +108,36 @@ class Simulation:
         if threads is not None:
             self.L.tmd_set_tile_threads(threads)
         check()
+
+    @classmethod
+    def restore(cls, path, sync=True):
+        """A simulation read from a restart file written by save()
+        (tmd_load_state): the saved atoms, geometry and modes, on the device;
+        ghosts and lists are built by the next setup()."""
+        self = cls.__new__(cls)
+        self.L = L = load()
+        L.tmd_clear_error()
+        L.md_deallocate_grid()
+        L.tmd_set_sync(int(bool(sync)))
+        L.tmd_set_adjoint_precision(-1)
+        self.n = L.tmd_load_state(os.fsencode(path))
+        check()
+        self.aabb = None
+        self.cell_spacing = None
+        return self
+
+    def save(self, path):
+        """Restart file of the current state (tmd_save_state)."""
+        n = self.L.tmd_save_state(os.fsencode(path))
+        check()
+        return n
+
+    def write_vtk(self, path, what="locals"):
+        """The reference's legacy VTK particle file of the host copy
+        (call copy_from_accelerator first): what = locals, ghosts or aabb."""
+        n = self.L.tmd_write_vtk(os.fsencode(path), {"locals": 0, "ghosts": 1, "aabb": 2}[what])
+        check()
+        return n
 
     @classmethod
     def fcc(cls, nx, ny=None, nz=None, potmin=1.6796, temp=1.44, half_benchmark=False, cutoff=2.5,

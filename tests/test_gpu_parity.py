@@ -458,3 +458,105 @@ def test_aabb_export(tm):
                 assert np.array_equal(pos[idx], want), (idx, pos[idx], want)
     assert np.all(mass == 0.0) and np.all(vel == 0.0) and np.all(frc == 0.0)
     sim.close()
+
+
+def parse_vtk(path):
+    """Points and the seven scalar fields of a legacy VTK file in the layout
+    of the reference's writer (vtk.h:6-91)."""
+    lines = open(path).read().split("\n")
+    assert lines[:4] == ["# vtk DataFile Version 2.0", "Particle data", "ASCII", "DATASET UNSTRUCTURED_GRID"]
+    n = int(lines[4].split()[1])
+    assert lines[4] == f"POINTS {n} double"
+    pts = np.array([[float(t) for t in ln.split()] for ln in lines[5:5 + n]]).reshape(n, 3)
+    k = 5 + n
+    assert lines[k:k + 3] == ["", "", f"CELLS {n} {2 * n}"]
+    assert lines[k + 3:k + 3 + n] == [f"1 {i}" for i in range(n)]
+    fields = {}
+    i = lines.index(f"POINT_DATA {n}")
+    while i + 1 < len(lines) and lines[i + 1].startswith("SCALARS"):
+        name = lines[i + 1].split()[1]
+        assert lines[i + 2] == "LOOKUP_TABLE default"
+        fields[name] = np.array([float(t) for t in lines[i + 3:i + 3 + n]])
+        i = i + 3 + n
+    return pts, fields
+
+
+def test_vtk_files(tm, tmp_path):
+    """tmd_write_vtk for locals, ghosts and the grid AABB: the reference's
+    layout, and every value equal to the exported arrays at six decimals
+    (the writer's std::fixed format)."""
+    announce("VTK files of locals, ghosts and AABB")
+    aabb, m, x, v = perturbed_fcc(5, amp=0.1, seed=21)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    sim.run_lj(7, 5, 0.005, False, RC, SKIN)
+    sim.copy_from_accelerator()
+    mg, xg, vg, fg = sim.state()
+    cases = {"locals": (mg, xg, vg, fg)}
+    gm, gx, gv, gf = sim.ghost_state()
+    cases["ghosts"] = (gm, gx, gv, gf)
+    for what, (mm, xx, vv, ff) in cases.items():
+        p = str(tmp_path / f"{what}.vtk")
+        assert sim.write_vtk(p, what) == len(mm)
+        pts, fields = parse_vtk(p)
+        r6 = lambda a: np.array([float(f"{t:.6f}") for t in np.ravel(a)]).reshape(np.shape(a))
+        assert np.array_equal(pts, r6(xx)), what
+        assert np.array_equal(fields["mass"], r6(mm))
+        for c, ax in enumerate("xyz"):
+            assert np.array_equal(fields["velocity_" + ax], r6(vv[:, c])), (what, ax)
+            assert np.array_equal(fields["force_" + ax], r6(ff[:, c])), (what, ax)
+    p = str(tmp_path / "aabb.vtk")
+    assert sim.write_vtk(p, "aabb") == 8
+    pts, fields = parse_vtk(p)
+    lo, hi = aabb[0::2] - (RC + SKIN), aabb[1::2] + (RC + SKIN)
+    for idx in range(8):
+        i, j, k = idx & 1, (idx >> 1) & 1, idx >> 2
+        corner = [lo[0] if i == 0 else hi[0], lo[1] if j == 0 else hi[1], lo[2] if k == 0 else hi[2]]
+        assert np.allclose(pts[idx], corner, atol=5e-7), (idx, pts[idx], corner)
+    sim.close()
+    orc.close()
+
+
+def test_restart_round_trip(tm, tmp_path):
+    """tmd_save_state / tmd_load_state: the restored arrays equal the saved
+    ones bit for bit, and a run continued from the file (lists rebuilt at
+    the start, as after md_initialize_grid) is bit-identical to the same
+    continuation of the original simulation."""
+    announce("restart file: save, restore, bit-identical continuation")
+    aabb, m, x, v = perturbed_fcc(6, amp=0.1, seed=22)
+    sim = tm.Simulation(m, x, v, aabb, RC + SKIN)
+    sim.copy_to_accelerator()
+    sim.setup(False, RC, SKIN)
+    sim.lennard_jones(False, RC)
+    sim.run_lj(30, 20, 0.005, False, RC, SKIN)
+    p = str(tmp_path / "state.tmdrst")
+    assert sim.save(p) == len(m)
+    _, x0, v0, f0 = sim.state()
+    # continue as the reference's driver starts a run: wrap, ghosts, cells,
+    # lists, forces (main.cpp:691-700)
+    sim.enforce_pbc()
+    sim.setup(False, RC, SKIN)
+    sim.lennard_jones(False, RC)
+    sim.run_lj(30, 20, 0.005, False, RC, SKIN)
+    _, xa, va, fa = sim.state()
+    sim.close()
+    # a corrupted copy is refused
+    raw = bytearray(open(p, "rb").read())
+    raw[200] ^= 1
+    bad = str(tmp_path / "bad.tmdrst")
+    open(bad, "wb").write(bytes(raw))
+    with pytest.raises(tm.TinyMDError, match="corrupted"):
+        tm.Simulation.restore(bad)
+    tm.load().tmd_clear_error()
+    sim2 = tm.Simulation.restore(p)
+    assert sim2.nlocal == len(m)
+    _, x1, v1, f1 = sim2.state()
+    assert np.array_equal(x1, x0) and np.array_equal(v1, v0) and np.array_equal(f1, f0)
+    sim2.enforce_pbc()
+    sim2.setup(False, RC, SKIN)
+    sim2.lennard_jones(False, RC)
+    sim2.run_lj(30, 20, 0.005, False, RC, SKIN)
+    _, xb, vb, fb = sim2.state()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(fa, fb)
+    sim2.close()
