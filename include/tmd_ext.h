@@ -70,6 +70,18 @@ void tmd_thermo(double rc, double epsilon, double sigma, double* out6);
  * is unchanged), for energy-drift bookkeeping.  Runs with the smooth cutoff
  * (tmd_set_lj_shift) report their own shifted energy either way. */
 void tmd_thermo_eshift(double rc, double epsilon, double sigma, double* out6);
+/* As tmd_thermo, without allocating: no hipMalloc, hipFree or allocating
+ * library call.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under stream capture, tight serving loops,
+ * harnesses that forbid allocation).  The reductions' partial sums live in a
+ * fixed 1 MiB device array that each GPU holds until the process ends, which
+ * limits the call to 3 * tiles + 8 <= 131072 and ceil(nlocal / 256) + 8 <=
+ * 131072 (about 43,000 tiles; an error is set beyond that).  The state must
+ * already be on the device and its buffers sized (md_copy_data_to_accelerator
+ * and a neighbour-list build); a call that would have to grow a buffer sets
+ * an error instead.  Calls that overlap on one GPU race on the array.
+ * Results are those of tmd_thermo, bit for bit. */
+void tmd_thermo_noalloc(double rc, double epsilon, double sigma, double* out6);
 
 /* ---- state I/O -------------------------------------------------------------
  * VTK: the reference's legacy particle file (vtk.h:6-91, written by main.cpp's

@@ -38,10 +38,20 @@ void set_error(const char* file, int line, const char* what) {
 
 bool has_error() { return g_has_error; }
 
+static int g_alloc_free_depth = 0;
+AllocFree::AllocFree() { ++g_alloc_free_depth; }
+AllocFree::~AllocFree() { --g_alloc_free_depth; }
+bool alloc_forbidden() { return g_alloc_free_depth > 0; }
+
 // ---------------------------------------------------------------------------
 // device buffers
 void DevBuf::reserve(size_t need, bool preserve, hipStream_t s) {
     if (need <= bytes || has_error()) return;
+    if (alloc_forbidden()) {
+        set_error(__FILE__, __LINE__, "allocation-free call: a device buffer would have to grow (set the state up "
+                                      "with the allocating entry points first)");
+        return;
+    }
     void* np = nullptr;
     TMD_HIP(hipMalloc(&np, need));
     if (has_error()) return;
@@ -273,6 +283,38 @@ static unsigned long long mix64(unsigned long long z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch memory of the allocation-free thermo entry point
+// (tmd_thermo_noalloc): the per-block partial sums of the kinetic and
+// energy/virial reductions and their totals, in a fixed-size device array
+// that every GPU the library runs on holds until the process ends.  Host
+// code never uses the array's name as an address: its device address is
+// looked up once per device (hipGetSymbolAddress, with that device current)
+// and kept in g_scratch.  Every reduction writes all the partials it reads,
+// so nothing needs clearing between calls; calls that overlap on one GPU
+// race on the array.
+constexpr size_t kScratchDoubles = size_t(1) << 17;  // 1 MiB
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static double* g_scratch[kMaxScratchDevices] = {};
+
+static double* scratch_ptr() {
+    int dev = -1;
+    TMD_HIP(hipGetDevice(&dev));
+    if (has_error()) return nullptr;
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        set_error(__FILE__, __LINE__, "allocation-free thermo: device index beyond the scratch table");
+        return nullptr;
+    }
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        TMD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (has_error()) return nullptr;
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    return g_scratch[dev];
 }
 
 }  // namespace tmd
@@ -873,6 +915,45 @@ static void thermo(double rc, double eps, double sigma, double* out, bool eshift
 void tmd_thermo(double rc, double eps, double sigma, double* out) { thermo(rc, eps, sigma, out, false); }
 
 void tmd_thermo_eshift(double rc, double eps, double sigma, double* out) { thermo(rc, eps, sigma, out, true); }
+
+void tmd_thermo_noalloc(double rc, double eps, double sigma, double* out) {
+    Sim& s = sim();
+    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    if (has_error()) return;
+    if (!s.stream || !s.dev_valid) {
+        set_error(__FILE__, __LINE__, "allocation-free thermo: the state is not on the device "
+                                      "(md_copy_data_to_accelerator first)");
+        return;
+    }
+    const int ntiles = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    const size_t need = std::max(3 * (size_t)std::max(ntiles, 0) + 8, (size_t)div_up(std::max(s.n, 1), 256) + 8);
+    if (need > kScratchDoubles) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "allocation-free thermo: %d atoms in %d tiles need %zu doubles of scratch, more "
+                                  "than the %zu it holds", s.n, ntiles, need, kScratchDoubles);
+        set_error(__FILE__, __LINE__, msg);
+        return;
+    }
+    double* buf = scratch_ptr();
+    if (!buf) return;
+    AllocFree guard;
+    // the same reductions as tmd_thermo, one after the other in the scratch
+    const double sum_mv2 = kinetic_sum(s, buf);
+    double pe = 0.0, vir = 0.0;
+    long long pairs = 0;
+    lj_energy_virial(s, rc, eps, sigma, &pe, &vir, &pairs, false, buf);
+    if (has_error()) return;
+    const double vol = s.g.len[0] * s.g.len[1] * s.g.len[2];
+    s.pe = pe;
+    s.virial = vir;
+    s.pairs = pairs;
+    out[0] = sum_mv2 * (1.0 / (double)(s.n * 3 - 3));
+    out[1] = 0.5 * sum_mv2;
+    out[2] = pe;
+    out[3] = vir;
+    out[4] = (sum_mv2 + vir) / (3.0 * vol);
+    out[5] = (double)pairs;
+}
 
 void tmd_get_grid_info(int* out) {
     Sim& s = sim();

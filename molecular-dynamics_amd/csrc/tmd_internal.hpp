@@ -18,6 +18,13 @@ namespace tmd {
 // printed, and reported by tmd_last_error(); later work is skipped.
 void set_error(const char* file, int line, const char* what);
 bool has_error();
+// While an AllocFree guard is alive, a DevBuf that would have to grow sets
+// an error instead of calling hipMalloc (the allocation-free entry points).
+bool alloc_forbidden();
+struct AllocFree {
+    AllocFree();
+    ~AllocFree();
+};
 
 #define TMD_HIP(call)                                                          \
     do {                                                                       \

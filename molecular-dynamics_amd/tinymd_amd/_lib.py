@@ -87,6 +87,7 @@ SIGNATURES = {
     "tmd_write_vtk": (_i, [ctypes.c_char_p, _i]),
     "tmd_save_state": (_i, [ctypes.c_char_p]),
     "tmd_load_state": (_i, [ctypes.c_char_p]),
+    "tmd_thermo_noalloc": (None, [_d, _d, _d, _dp]),
     "tmd_get_grid_info": (None, [_ip]),
     "tmd_get_ghost_table": (_i, [_ip, _ip, _ip, _ip]),
     "tmd_get_cells": (_i, [_ip, _ip, _ip]),
