@@ -35,6 +35,15 @@ void tmd_set_precision(int mode);
  * forward precision separates the adjoint's own rounding from that of the
  * FP32-mixed forward trajectory. */
 void tmd_set_adjoint_precision(int mode);
+/* 1: renumber the local atoms in cell order (index order inside a cell) at
+ * every periodic wrap of a rebuild (md_enforce_pbc, and inside tmd_run_*),
+ * before the ghosts are built; positions, velocities, forces and masses move
+ * with them, and the arrays written by md_write_grid_data_to_arrays follow
+ * the new numbering.  This is the reference's sort_particles() hook
+ * (core/grid.impala:54, 323-356; compiled out there), taken before borders
+ * so that the ghost layer refers to the new numbering.  0 (default): atoms
+ * keep their indices, as in the reference's build. */
+void tmd_set_sort_particles(int on);
 /* Half-list filter.  0 = local j < i plus every ghost (each periodic pair
  * once; default), 1 = the reference's literal i > j filter, which drops all
  * ghost pairs (core/neighborlist.impala:34). */

@@ -242,19 +242,11 @@ static void choose_tiles(Sim& s) {
     s.tiles = make_tile_geom(s.g, t[0], t[1], t[2]);
 }
 
-void bin_particles(Sim& s) {
-    if (has_error()) return;
-    const int nall = s.n + s.ng;
+// Counting sort of atoms 0..nall-1 by cell: cell_count, cell_start (CSR),
+// cell_parts (index order inside each cell), cell_of, cell_inv.
+static void bin_counting_sort(Sim& s, int nall) {
     const Geom& G = s.g;
-    s.cell_count.reserve(sizeof(int) * (size_t)G.ncells, false, s.stream);
-    s.cell_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
-    s.work.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
-    s.work_k.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
-    s.work_c.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
-    if (has_error()) return;
     BinArgs a{G.aabb[0], G.aabb[2], G.aabb[4], G.spacing, G.nx, G.ny, G.ncells, nall};
-    mirror_invalidate(s);
-    s.inv_valid = 0;
     const int nt = 256;
     TMD_HIP(hipMemsetAsync(s.cell_count.p, 0, sizeof(int) * (size_t)G.ncells, s.stream));
     if (nall > 0)
@@ -272,6 +264,88 @@ void bin_particles(Sim& s) {
         hipLaunchKernelGGL(k_bin_sort, dim3(div_up(G.ncells, nt / kWave)), dim3(nt), 0, s.stream,
                            (const int*)s.cell_start.as<int>(), G.ncells, s.n, s.cell_parts.as<int>(), s.cell_count.as<int>(),
                            s.cell_inv.as<int>());
+}
+
+// k-th local of the cell order <- local parts[k]: positions, velocities,
+// forces and masses into the scratch copies
+__global__ void k_permute_locals(int n, const int* __restrict__ parts, const double4* __restrict__ pos,
+                                 const double4* __restrict__ vel, const double4* __restrict__ force,
+                                 const double* __restrict__ mass, double4* __restrict__ pos2, double4* __restrict__ vel2,
+                                 double4* __restrict__ force2, double* __restrict__ mass2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = parts[k];
+    pos2[k] = pos[i];
+    vel2[k] = vel[i];
+    force2[k] = force[i];
+    mass2[k] = mass[i];
+}
+
+// Cell-order reordering of the locals (tmd_set_sort_particles; the
+// reference's sort_particles() hook, core/grid.impala:54, 323-356): at the
+// PBC wrap of a rebuild, before the ghosts are built, the locals are
+// renumbered in cell order, index order inside a cell (a stable counting
+// sort), and their positions, velocities, forces and masses move with them.
+// The ghost layer built next refers to the new numbering.  (The reference's
+// hook renumbers inside distribute_particles, after borders, and copies only
+// positions and velocities, so its ghosts keep the old source indices; it is
+// compiled out there.)  A local outside the grid after the wrap is an error,
+// as in binning.
+void sort_locals(Sim& s) {
+    if (has_error() || s.n == 0) return;
+    // the fused step loop keeps positions and velocities in the mirrors
+    sync_pos_from_mirror(s);
+    enforce_pbc(s);
+    const Geom& G = s.g;
+    s.cell_count.reserve(sizeof(int) * (size_t)G.ncells, false, s.stream);
+    s.cell_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
+    const size_t n = (size_t)s.n;
+    s.sort_buf.reserve((sizeof(double4) * 3 + sizeof(double)) * n, false, s.stream);
+    if (has_error()) return;
+    s.ng = 0;
+    bin_counting_sort(s, s.n);
+    int binned = 0;
+    TMD_HIP(hipMemcpyAsync(&binned, s.cell_start.as<int>() + G.ncells, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    TMD_HIP(hipStreamSynchronize(s.stream));
+    if (has_error()) return;
+    if (binned != s.n) {
+        set_error(__FILE__, __LINE__, "sort: local atoms outside the cell grid after the periodic wrap (non-finite positions?)");
+        return;
+    }
+    double4* p2 = s.sort_buf.as<double4>();
+    double4* v2 = p2 + n;
+    double4* f2 = v2 + n;
+    double* m2 = reinterpret_cast<double*>(f2 + n);
+    hipLaunchKernelGGL(k_permute_locals, dim3(div_up(s.n, 256)), dim3(256), 0, s.stream, s.n, (const int*)s.cell_parts.as<int>(),
+                       (const double4*)s.pos.as<double4>(), (const double4*)s.vel.as<double4>(),
+                       (const double4*)s.force.as<double4>(), (const double*)s.mass.as<double>(), p2, v2, f2, m2);
+    TMD_HIP(hipMemcpyAsync(s.pos.p, p2, sizeof(double4) * n, hipMemcpyDeviceToDevice, s.stream));
+    TMD_HIP(hipMemcpyAsync(s.vel.p, v2, sizeof(double4) * n, hipMemcpyDeviceToDevice, s.stream));
+    TMD_HIP(hipMemcpyAsync(s.force.p, f2, sizeof(double4) * n, hipMemcpyDeviceToDevice, s.stream));
+    TMD_HIP(hipMemcpyAsync(s.mass.p, m2, sizeof(double) * n, hipMemcpyDeviceToDevice, s.stream));
+    TMD_HIP(hipGetLastError());
+    // every atom-indexed table is stale now
+    mirror_invalidate(s);
+    s.inv_valid = 0;
+    s.gsrc_valid = 0;
+    s.windows_valid = 0;
+    s.nb_valid = 0;
+}
+
+void bin_particles(Sim& s) {
+    if (has_error()) return;
+    const int nall = s.n + s.ng;
+    const Geom& G = s.g;
+    s.cell_count.reserve(sizeof(int) * (size_t)G.ncells, false, s.stream);
+    s.cell_start.reserve(sizeof(int) * ((size_t)G.ncells + 1), false, s.stream);
+    s.work.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
+    s.work_k.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
+    s.work_c.reserve(sizeof(int) * ((size_t)s.n + 1), false, s.stream);
+    if (has_error()) return;
+    mirror_invalidate(s);
+    s.inv_valid = 0;
+    const int nt = 256;
+    bin_counting_sort(s, nall);
     s.inv_valid = 1;
     build_ghost_src(s);
     // tile work order

@@ -252,6 +252,11 @@ static double host_temperature(const Sim& s) {
 static void rebuild(Sim& s, int half, double cutoff) {
     s.ng = 0;  // exchange_particles on one rank (comm/comm.impala:392-396)
     hipEvent_t ev = nullptr;
+    if (s.sort_particles) {
+        s.timer.begin(K_BIN, s.stream, &ev);
+        sort_locals(s);  // wraps, then renumbers the locals in cell order
+        s.timer.end(K_BIN, s.stream, ev);
+    }
     s.timer.begin(K_BORDERS, s.stream, &ev);
     build_ghosts(s, true);  // md_enforce_pbc + borders
     s.timer.end(K_BORDERS, s.stream, ev);
@@ -416,7 +421,12 @@ void md_final_integration(double dt) {
 void md_enforce_pbc() {
     Sim& s = sim();
     if (!ready(s)) return;
-    enforce_pbc(s);
+    if (s.sort_particles) {
+        sort_locals(s);  // the wrap, then the locals in cell order (the ghosts are rebuilt next)
+        s.ng = 0;
+    } else {
+        enforce_pbc(s);
+    }
     finish(s);
 }
 
@@ -651,7 +661,7 @@ void md_deallocate_grid() {
     Sim& s = *g_sim;
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
-    DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
+    DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.sort_buf, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
                       &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.g_ko, &s.g_kg, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel,
                       &s.nb_count, &s.nb_list, &s.nb_count2, &s.nb_list2,
                       &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
@@ -867,6 +877,7 @@ void tmd_set_ghost_mode(int mode) { sim().ghost_mode = mode ? GHOST_REF_LAG : GH
 void tmd_set_sync(int on) { sim().sync_calls = on ? 1 : 0; }
 void tmd_set_dem_radius(double r) { sim().dem_radius = r; }
 void tmd_set_lj_shift(int on) { sim().lj_shift = on ? 1 : 0; }
+void tmd_set_sort_particles(int on) { sim().sort_particles = on ? 1 : 0; }
 void tmd_set_adjoint_precision(int mode) { sim().adj_precision = mode < 0 ? -1 : (mode ? PREC_FP32_MIXED : PREC_FP64); }
 
 void tmd_set_tile_threads(int threads) {

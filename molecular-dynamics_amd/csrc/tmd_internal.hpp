@@ -117,6 +117,7 @@ struct Sim {
     int precision = PREC_FP64;
     int adj_precision = -1;  // adjoint kernel precision (-1: the forward's)
     int half_mode = HALF_GHOST_FULL;
+    int sort_particles = 0;  // renumber the locals in cell order at each wrap (sort_locals)
     int ghost_mode = GHOST_FRESH;
     int sync_calls = 1;
     double dem_radius = 0.0;  // <= 0: contact cutoff / 2
@@ -130,6 +131,7 @@ struct Sim {
 
     // device state
     DevBuf pos, vel, force, mass;      // double4 x3, double
+    DevBuf sort_buf;                   // sort_locals: scratch copies of the four arrays above
     DevBuf g_owner, g_src, g_code;     // ghost table
     DevBuf cell_of, cell_slot, cell_count, cell_start, cell_parts;
     DevBuf cell_inv;                   // CSR index of each atom (-1: outside the grid)
@@ -197,6 +199,7 @@ void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell
 
 // binning
 void bin_particles(Sim& s);
+void sort_locals(Sim& s);  // PBC wrap + renumber the locals in cell order
 int auto_tile_edge(const Geom& G, int n);  // cells per tile edge for ~480 locals per tile
 TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz);
 

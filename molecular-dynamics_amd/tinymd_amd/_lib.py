@@ -75,6 +75,7 @@ SIGNATURES = {
     "tmd_kahan_sum": (_d, [ctypes.POINTER(ctypes.c_float), _i]),
     "tmd_set_precision": (None, [_i]),
     "tmd_set_adjoint_precision": (None, [_i]),
+    "tmd_set_sort_particles": (None, [_i]),
     "tmd_set_half_mode": (None, [_i]),
     "tmd_set_ghost_mode": (None, [_i]),
     "tmd_set_sync": (None, [_i]),

@@ -64,11 +64,12 @@ def device_info():
 class Simulation:
     def __init__(self, masses, positions, velocities, aabb, cell_spacing, rank_aabb=None, nb_cap=100,
                  cell_cap=60, precision="fp64", half_mode="ghost_full", ghost_mode="fresh", lj_shift=False,
-                 dem_radius=None, sync=True):
+                 dem_radius=None, sync=True, sort=False):
         self.L = L = load()
         L.tmd_clear_error()
         L.md_deallocate_grid()
         self.set_modes(precision, half_mode, ghost_mode, lj_shift, dem_radius, sync)
+        L.tmd_set_sort_particles(int(bool(sort)))
         m = np.ascontiguousarray(masses, dtype=np.float64)
         x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         v = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
@@ -120,6 +121,7 @@ class Simulation:
         L.md_deallocate_grid()
         L.tmd_set_sync(int(bool(sync)))
         L.tmd_set_adjoint_precision(-1)
+        L.tmd_set_sort_particles(0)
         self.n = L.tmd_load_state(os.fsencode(path))
         check()
         self.aabb = None

@@ -54,6 +54,7 @@ def lib():
             "orc_create_velocity": (None, [_vp, _d]),
             "orc_exchange": (None, [_vp]),
             "orc_enforce_pbc": (None, [_vp]),
+            "orc_set_sort": (None, [_vp, _i]),
             "orc_borders": (None, [_vp]),
             "orc_distribute": (None, [_vp]),
             "orc_neighbors": (None, [_vp, _i, _i, _d]),
@@ -153,6 +154,10 @@ class OracleSim:
 
     def set_ghost_mode(self, mode):
         self.L.orc_set_ghost_mode(self.h, mode)
+
+    def set_sort(self, on=True):
+        """Renumber the locals in cell order at each wrap (orc_set_sort)."""
+        self.L.orc_set_sort(self.h, int(bool(on)))
 
     def create_velocity(self, t):
         self.L.orc_create_velocity(self.h, t)

@@ -99,6 +99,7 @@ struct Sim {
     int nb_half_mode = HALF_GHOST_FULL;
 
     int ghost_mode = GHOST_FRESH;
+    int sort = 0;  // renumber the locals in cell order at each wrap (sort_locals)
     int threads = 1;
 
     // last force evaluation (full list: 1/2 per pair entry; half list:
@@ -536,11 +537,46 @@ void enforce_pbc(Sim* s) {
     for (int i = 0; i < s->n; ++i) pbc_correct(s, &s->x[3 * i]);
 }
 
+// The locals renumbered in cell order, index order inside a cell (a stable
+// counting sort by the cell of core/grid.impala:406-426), positions,
+// velocities, forces and masses moving together.  This is the reference's
+// sort_particles() reordering (core/grid.impala:323-356), taken before
+// borders(): the ghosts are built from the new numbering.  Locals only;
+// call with no ghosts present.
+void sort_locals(Sim* s) {
+    const int n = s->n;
+    // a local outside the grid (not after a wrap) goes last, in index order
+    std::vector<int> cell(n), count(s->ncells + 2, 0);
+    for (int i = 0; i < n; ++i) {
+        const int c = cell_of(s, &s->x[3 * i]);
+        cell[i] = (c >= 0 && c < s->ncells) ? c : s->ncells;
+        count[cell[i] + 1]++;
+    }
+    for (int c = 0; c <= s->ncells; ++c) count[c + 1] += count[c];
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[count[cell[i]]++] = i;
+    std::vector<double> m(n), x(3 * n), v(3 * n), f(3 * n);
+    for (int k = 0; k < n; ++k) {
+        const int i = order[k];
+        m[k] = s->m[i];
+        for (int e = 0; e < 3; ++e) {
+            x[3 * k + e] = s->x[3 * i + e];
+            v[3 * k + e] = s->v[3 * i + e];
+            f[3 * k + e] = s->f[3 * i + e];
+        }
+    }
+    std::copy(m.begin(), m.end(), s->m.begin());
+    std::copy(x.begin(), x.end(), s->x.begin());
+    std::copy(v.begin(), v.end(), s->v.begin());
+    std::copy(f.begin(), f.end(), s->f.begin());
+}
+
 void rebuild(Sim* s, int half, int half_mode, double cutoff) {
     // exchange_particles (single rank): nghost = 0.  comm/comm.impala:392-396
     s->ng = 0;
     s->m.resize(s->n); s->x.resize(3 * s->n); s->v.resize(3 * s->n);
     enforce_pbc(s);
+    if (s->sort) sort_locals(s);
     borders(s);
     distribute(s);
     neighbors(s, half, half_mode, cutoff);
@@ -612,7 +648,17 @@ void orc_exchange(void* h) {
     s->ng = 0;
     s->m.resize(s->n); s->x.resize(3 * s->n); s->v.resize(3 * s->n);
 }
-void orc_enforce_pbc(void* h) { enforce_pbc((Sim*)h); }
+void orc_enforce_pbc(void* h) {
+    Sim* s = (Sim*)h;
+    enforce_pbc(s);
+    if (s->sort) {
+        // the ghosts are rebuilt after the wrap (md_borders)
+        s->ng = 0;
+        s->m.resize(s->n); s->x.resize(3 * s->n); s->v.resize(3 * s->n);
+        sort_locals(s);
+    }
+}
+void orc_set_sort(void* h, int on) { ((Sim*)h)->sort = on ? 1 : 0; }
 void orc_borders(void* h) { borders((Sim*)h); }
 void orc_distribute(void* h) { distribute((Sim*)h); }
 void orc_neighbors(void* h, int half, int half_mode, double cutoff) { neighbors((Sim*)h, half, half_mode, cutoff); }

@@ -18,6 +18,7 @@ void orc_grid_info(void* h, int* out);
 void orc_create_velocity(void* h, double t_request);
 void orc_exchange(void* h);
 void orc_enforce_pbc(void* h);
+void orc_set_sort(void* h, int on);
 void orc_borders(void* h);
 void orc_distribute(void* h);
 void orc_neighbors(void* h, int half, int half_mode, double cutoff);

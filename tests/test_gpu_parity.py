@@ -560,3 +560,38 @@ def test_restart_round_trip(tm, tmp_path):
     _, xb, vb, fb = sim2.state()
     assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(fa, fb)
     sim2.close()
+
+
+@pytest.mark.parametrize("loop", ["abi", "runner"])
+def test_sort_particles_trajectory_bitwise(tm, loop):
+    """Cell-order renumbering of the locals at each rebuild (sort mode), on
+    the GPU and in the oracle: FP64 trajectories bit-exact, through the md_*
+    driver loop (the sort happens in md_enforce_pbc) and the library's own
+    loop; right after a rebuild the exported locals are in cell order."""
+    announce(f"sort mode 7^3, 40 + 5 steps, rebuild every 20, loop={loop}")
+    aabb, m, x, v = perturbed_fcc(7, amp=0.1, seed=33)
+    sim, orc = make_pair(aabb, m, x, v, sort=True)
+    orc.set_sort(True)
+    setup_both(sim, orc)
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    run = sim.run_abi_lj if loop == "abi" else sim.run_lj
+    run(40, reneigh=20, dt=0.005)
+    orc.run_lj(40, 20, 0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    # the last step rebuilt (and sorted) after its drift: cell order now
+    cut = RC + SKIN
+    lo = aabb[0::2] - cut
+    inf = orc.info()
+    ijk = np.floor((xg - lo) / cut).astype(np.int64)
+    key = (ijk[:, 2] * inf["ny"] + ijk[:, 1]) * inf["nx"] + ijk[:, 0]
+    assert np.all(np.diff(key) >= 0)
+    run(5, reneigh=20, dt=0.005)
+    orc.run_lj(5, 20, 0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    sim.close()
+    orc.close()

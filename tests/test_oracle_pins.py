@@ -418,3 +418,49 @@ def test_dem_tangential_coulomb_cap():
         assert abs(f[0, 0] - (-fn)) < 1e-12
         assert abs(f[0, 1] - min(gt * 2.0, mu * fn)) < 1e-12  # i is dragged along +y
         assert np.allclose(f[0], -f[1], atol=1e-14)
+
+
+def test_sort_locals_cell_order_and_same_physics():
+    """The oracle's cell-order renumbering (sort mode): after a rebuild the
+    locals are ordered by cell, index order inside a cell (the stable order
+    a counting sort gives, computed here with numpy from the cell formula of
+    core/grid.impala:406-426); the positions, velocities and forces are the
+    unsorted run's, permuted (forces to 1e-12: the ghost layer is built
+    from the new numbering, so summation orders differ)."""
+    aabb, m, x, v = po.lattice(6, 5, 7)
+    rng = np.random.default_rng(31)
+    L = aabb[1::2] - aabb[0::2]
+    x = np.mod(x + rng.uniform(-0.1, 0.1, x.shape), L)
+    cut = RC + SKIN
+    a = po.OracleSim(m, x, v, aabb, cut)
+    b = po.OracleSim(m, x, v, aabb, cut)
+    b.set_sort(True)
+    a.create_velocity(1.44)
+    b.create_velocity(1.44)
+    for s in (a, b):
+        s.enforce_pbc()
+        s.setup(False, po.HALF_GHOST_FULL, cut)
+        s.lj(False, RC)
+    xa, va, fa = a.state()
+    xb, vb, fb = b.state()
+    lo = aabb[0::2] - cut
+    inf = a.info()
+    ijk = np.floor((xa - lo) / cut).astype(np.int64)
+    key = (ijk[:, 2] * inf["ny"] + ijk[:, 1]) * inf["nx"] + ijk[:, 0]
+    order = np.argsort(key, kind="stable")
+    assert np.array_equal(xb, xa[order]) and np.array_equal(vb, va[order])
+    rms = np.sqrt(np.mean(np.sum(fa * fa, axis=1)))
+    assert np.abs(fb - fa[order]).max() <= 1e-12 * rms
+    kb = key[order]
+    assert np.all(np.diff(kb) >= 0)
+    # a short run with rebuilds: still the same trajectory up to the labels
+    a.run_lj(25, 10, 0.005)
+    b.run_lj(25, 10, 0.005)
+    xa, _, _ = a.state()
+    xb, _, _ = b.state()
+    # both runs wrap at every rebuild; match atoms by their nearest periodic image
+    from scipy.spatial import cKDTree
+    d, idx = cKDTree(np.mod(xa, L), boxsize=L).query(np.mod(xb, L))
+    assert d.max() < 1e-9 and len(set(idx.tolist())) == len(idx)
+    a.close()
+    b.close()
