@@ -201,10 +201,12 @@ int tmd_batch_setup(int handle, double cutoff, double skin, double epsilon, doub
  * count, ghost room headroom over the expected ghost count. */
 int tmd_batch_configure(int handle, int tile, int threads, double spacing_factor, double list_headroom,
                         double ghost_headroom);
-/* Times trial runs of the candidate configurations from the initial state,
- * keeps the fastest and restarts from the initial state with it.  out
- * (may be NULL): tile, threads in use, spacing factor, list headroom,
- * ms/step. */
+/* Times trial runs of candidate configurations from the initial state, in
+ * two stages: cells per tile edge (automatic, 2, 3, 4) x threads per block
+ * (automatic, 256, 512) at cell edge 1.0 and list headroom 1.1, then cell
+ * edge (1.0, 1.1) x list headroom (1.1, 1.3) at the best of those; keeps
+ * the fastest and restarts from the initial state with it.  out (may be
+ * NULL): tile, threads in use, spacing factor, list headroom, ms/step. */
 int tmd_batch_autotune(int handle, int trial_steps, double* out5);
 /* Restart from the initial state with the current configuration. */
 int tmd_batch_reset(int handle);

@@ -1415,32 +1415,40 @@ int tmd_batch_autotune(int h, int trial_steps, double* out) {
     if (!B || has_error()) return 0;
     if (trial_steps <= 0) trial_steps = 2 * B->reneigh;
     const BatchCfg keep = B->cfg;
-    const int tiles[] = {0, 2, 3, 4};
-    const int threads[] = {0, 256, 512};  // 0: one lane per local of the fullest tile
-    const double spacing[] = {1.0, 1.1};
-    const double headroom[] = {1.1, 1.3};
     double best = 1e300;
     BatchCfg best_cfg = keep;
+    // one trial: set up from the initial state, a warm run (graph capture),
+    // the same start again, a timed run
+    auto trial = [&](int t, int th, double sp, double hr) -> bool {
+        B->cfg = keep;
+        B->cfg.tile = t;
+        B->cfg.threads = th;
+        B->cfg.spacing = sp;
+        B->cfg.list_headroom = hr;
+        B->ghost_scale = 1.0;
+        double ms = 0.0;
+        if (!setup_from_initial(*B) || !run_steps(*B, trial_steps, nullptr)) return false;
+        if (!setup_from_initial(*B) || !run_steps(*B, trial_steps, &ms)) return false;
+        if (ms < best) {
+            best = ms;
+            best_cfg = B->cfg;
+        }
+        return true;
+    };
+    // stage 1: the launch shape (cells per tile edge x threads per block) at
+    // the reference's cell edge and the smaller list headroom; stage 2: cell
+    // edge and list headroom around the best shape (15 trials, not 48)
+    const int tiles[] = {0, 2, 3, 4};
+    const int threads[] = {0, 256, 512};  // 0: one lane per local of the fullest tile
+    for (int t : tiles)
+        for (int th : threads)
+            if (!trial(t, th, 1.0, 1.1)) return 0;
+    const BatchCfg shape = best_cfg;
+    const double spacing[] = {1.0, 1.1};
+    const double headroom[] = {1.1, 1.3};
     for (double sp : spacing)
-        for (int t : tiles)
-            for (int th : threads)
-                for (double hr : headroom) {
-                    B->cfg = keep;
-                    B->cfg.tile = t;
-                    B->cfg.threads = th;
-                    B->cfg.spacing = sp;
-                    B->cfg.list_headroom = hr;
-                    B->ghost_scale = 1.0;
-                    if (!setup_from_initial(*B)) return 0;
-                    double ms = 0.0;
-                    if (!run_steps(*B, trial_steps, nullptr)) return 0;  // warm (graph capture)
-                    if (!setup_from_initial(*B)) return 0;
-                    if (!run_steps(*B, trial_steps, &ms)) return 0;
-                    if (ms < best) {
-                        best = ms;
-                        best_cfg = B->cfg;
-                    }
-                }
+        for (double hr : headroom)
+            if (!(sp == 1.0 && hr == 1.1) && !trial(shape.tile, shape.threads, sp, hr)) return 0;
     B->cfg = best_cfg;
     B->ghost_scale = 1.0;
     if (!setup_from_initial(*B)) return 0;

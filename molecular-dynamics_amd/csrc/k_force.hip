@@ -314,6 +314,174 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS,
 }
 
 // ---------------------------------------------------------------------------
+// DEM step loop with the state in the cell-ordered mirrors (spos, svel,
+// ghosts included).  The block stages its window's positions and
+// velocities into LDS (48 bytes per atom), so a contact's velocity is an LDS
+// read rather than a global gather; the lane reads its own atom at its mirror
+// index.  With ADV the pass ends the step (final kick v += f dt/2) and runs
+// the next step's kick and drift, writing the new position and velocity
+// into the other mirrors (spos_next, svel_next; other blocks still read the
+// current ones); without it (the last step) it stores the force, the
+// velocity and the position in atom order.  The pair arithmetic, the
+// contact decisions and the summation order are k_dem's, and the kicks are
+// k_dem_finish's, so the trajectory is that of the separate passes.
+constexpr int kDemStageBytes = 48;
+
+static int dem_window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kDemStageBytes); }
+
+template <int PREC, bool PAIR2, int ADV>
+__global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const double4* __restrict__ spos, const double4* __restrict__ svel,
+                                                           const int* __restrict__ start, const int* __restrict__ work,
+                                                           const int* __restrict__ wk, const int* __restrict__ tile_start,
+                                                           const int* __restrict__ nbc, const nb_t* __restrict__ nbl,
+                                                           const int* __restrict__ nbc2, const nb_t* __restrict__ nbl2,
+                                                           DEMArgs a, double hdt, double dt, double4* __restrict__ spos_next,
+                                                           double4* __restrict__ svel_next, double4* __restrict__ force,
+                                                           double4* __restrict__ vel, double4* __restrict__ pos) {
+    extern __shared__ double dyn[];
+    const int tile = blockIdx.x;
+    const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
+    if (nloc == 0) return;
+    __shared__ TileWin W;
+    tile_window(tile, start, a.g, W);
+    double* sx = dyn;
+    double* sy = dyn + a.wcap;
+    double* sz = dyn + 2 * a.wcap;
+    double* su = dyn + 3 * a.wcap;
+    double* sv = dyn + 4 * a.wcap;
+    double* sw = dyn + 5 * a.wcap;
+    const bool staged = win_size(W) <= a.wcap;
+    if (staged) {
+        // one thread group per window row, as in stage_window
+        const int nseg = W.nseg;
+        const int gsz = max(1, (int)blockDim.x / nseg);
+        const int ngrp = (int)blockDim.x / gsz;
+        const int g = (int)threadIdx.x / gsz, l = (int)threadIdx.x - g * gsz;
+        if (g < ngrp) {
+            for (int sg = g; sg < nseg; sg += ngrp) {
+                const int off = W.seg_off[sg], beg = W.seg_beg[sg], len = W.seg_off[sg + 1] - off;
+                for (int q = l; q < len; q += gsz) {
+                    const int t = off + q, k = beg + q;
+                    const double4 p = spos[k];
+                    const double4 u = svel[k];
+                    sx[t] = p.x; sy[t] = p.y; sz[t] = p.z;
+                    su[t] = u.x; sv[t] = u.y; sw[t] = u.z;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    auto atoms = [&](auto staged_c) {
+        constexpr bool STAGED = decltype(staged_c)::value;
+        for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
+            const int w = w0 + at;
+            const int kw = wk[w];
+            const double4 pi = spos[kw];
+            const double4 vi = svel[kw];
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            auto entries = [&](const nb_t* __restrict__ L, int cnt) {
+                const int4* lp = reinterpret_cast<const int4*>(L + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
+                int4 e = make_int4(0, 0, 0, 0);
+                for (int k = 0; k < cnt; ++k) {
+                    if ((k & 7) == 0) e = ld_stream(lp + (size_t)(k >> 3) * 64);
+                    const int t = (k & 1) ? (int)((unsigned)e.x >> 16) : (e.x & 0xffff);
+                    if (k & 1) e = make_int4(e.y, e.z, e.w, 0);
+                    double xj, yj, zj, uj, vj, wj;
+                    if (STAGED) {
+                        xj = sx[t]; yj = sy[t]; zj = sz[t];
+                        uj = su[t]; vj = sv[t]; wj = sw[t];
+                    } else {
+                        const int kk = win_index(t, W);
+                        const double4 p = spos[kk];
+                        const double4 u = svel[kk];
+                        xj = p.x; yj = p.y; zj = p.z;
+                        uj = u.x; vj = u.y; wj = u.z;
+                    }
+                    const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                    const double rsq = dx * dx + dy * dy + dz * dz;
+                    const double dd = a.d2 * a.d2 - rsq;
+                    if (rsq < a.rc2 && dd > 0.0) {
+                        double Fx, Fy, Fz;
+                        if (PREC == PREC_FP64) {
+                            dem_pair<double>(a, dx, dy, dz, rsq, dd, vi.x - uj, vi.y - vj, vi.z - wj, Fx, Fy, Fz);
+                        } else {
+                            float gx, gy, gz;
+                            dem_pair<float>(a, (float)dx, (float)dy, (float)dz, (float)rsq, (float)dd, (float)(vi.x - uj),
+                                            (float)(vi.y - vj), (float)(vi.z - wj), gx, gy, gz);
+                            Fx = gx; Fy = gy; Fz = gz;
+                        }
+                        fx = fx + Fx; fy = fy + Fy; fz = fz + Fz;
+                    }
+                }
+            };
+            entries(nbl, nbc[w]);
+            if (PAIR2) entries(nbl2, nbc2[w]);
+            // k_dem_finish's arithmetic
+            double4 v = vi;
+            v.x = v.x + fx * hdt;
+            v.y = v.y + fy * hdt;
+            v.z = v.z + fz * hdt;
+            if (ADV) {
+                v.x = v.x + fx * hdt;
+                v.y = v.y + fy * hdt;
+                v.z = v.z + fz * hdt;
+                double4 x = pi;
+                x.x = x.x + v.x * dt;
+                x.y = x.y + v.y * dt;
+                x.z = x.z + v.z * dt;
+                spos_next[kw] = x;
+                svel_next[kw] = v;
+            } else {
+                const int i = work[w];
+                force[i] = make_double4(fx, fy, fz, 0.0);
+                vel[i] = v;
+                pos[i] = pi;
+            }
+        }
+    };
+    if (staged) atoms(std::true_type{});
+    else atoms(std::false_type{});
+}
+
+void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double dt, bool advance) {
+    if (has_error()) return;
+    TMD_REQUIRE(s.nb_valid && s.dem_mirrored, "DEM step loop: the state is not in the mirrors");
+    TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
+    if (has_error() || s.n == 0) return;
+    const double radius = s.dem_radius > 0.0 ? s.dem_radius : 0.5 * rc;
+    s.last_force_rc = rc;
+    DEMArgs a{rc * rc, 2.0 * radius, gn, gt, k, mu, s.nb_cap, s.n, s.n, dem_window_cap(s), s.tiles};
+    const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    const size_t lds = (size_t)a.wcap * kDemStageBytes;
+    const double hdt = 0.5 * dt;
+    hipEvent_t ev = nullptr;
+    s.timer.begin(K_FORCE, s.stream, &ev);
+#define TMD_DEMF(P, H, A)                                                                                               \
+    hipLaunchKernelGGL((k_dem_fused<P, H, A>), dim3(nb), dim3(s.tile_threads), lds, s.stream,                            \
+                       (const double4*)s.spos.as<double4>(), (const double4*)s.svel.as<double4>(),                     \
+                       (const int*)s.cell_start.as<int>(), (const int*)s.work.as<int>(), (const int*)s.work_k.as<int>(), \
+                       (const int*)s.tile_start.as<int>(), (const int*)s.nb_count.as<int>(), (const nb_t*)s.nb_list.as<nb_t>(), \
+                       (const int*)s.nb_count2.as<int>(), (const nb_t*)s.nb_list2.as<nb_t>(), a, hdt, dt,               \
+                       s.spos2.as<double4>(), s.svel2.as<double4>(), s.force.as<double4>(), s.vel.as<double4>(),          \
+                       s.pos.as<double4>())
+#define TMD_DEMF_A(P, H) do { if (advance) TMD_DEMF(P, H, 1); else TMD_DEMF(P, H, 0); } while (0)
+#define TMD_DEMF_H(P) do { if (half) TMD_DEMF_A(P, true); else TMD_DEMF_A(P, false); } while (0)
+    if (s.precision == PREC_FP64) TMD_DEMF_H(PREC_FP64);
+    else TMD_DEMF_H(PREC_FP32_MIXED);
+#undef TMD_DEMF_H
+#undef TMD_DEMF_A
+#undef TMD_DEMF
+    s.timer.end(K_FORCE, s.stream, ev);
+    TMD_HIP(hipGetLastError());
+    if (advance) {
+        std::swap(s.spos, s.spos2);
+        std::swap(s.svel, s.svel2);
+    } else {
+        // the locals are back in atom order; their ghosts follow from them
+        s.dem_mirrored = 0;
+        refresh_ghosts(s);
+    }
+}
 
 #define TMD_LIST_ARGS                                                                                              \
     (const double4*)s.pos.as<double4>(), (const double4*)s.spos.as<double4>(), (const int*)s.cell_start.as<int>(), \

@@ -306,6 +306,38 @@ __global__ void k_ghost_step_mirror(const int* __restrict__ g_ko, const int* __r
     spos[kg] = q;
 }
 
+// DEM step loop: the ghosts' mirror entries from their owners' (position +
+// image shift, velocity copied), k_ghost_refresh's arithmetic.
+__global__ void k_ghost_step_mirror_v(const int* __restrict__ g_ko, const int* __restrict__ g_kg,
+                                      const int* __restrict__ g_code, int ng, double L0, double L1, double L2,
+                                      double4* __restrict__ spos, double4* __restrict__ svel) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int kg = g_kg[g];
+    if (kg < 0) return;
+    int ix, iy, iz;
+    img_decode(g_code[g] & 0xff, ix, iy, iz);
+    const int ko = g_ko[g];
+    const double4 p = spos[ko];
+    double4 q;
+    q.x = p.x + L0 * (double)ix;
+    q.y = p.y + L1 * (double)iy;
+    q.z = p.z + L2 * (double)iz;
+    q.w = p.w;
+    spos[kg] = q;
+    svel[kg] = svel[ko];
+}
+
+void dem_ghost_step(Sim& s) {
+    if (has_error() || s.ng == 0) return;
+    if (!s.gsrc_valid) build_ghost_src(s);
+    const int nt = 256;
+    hipLaunchKernelGGL(k_ghost_step_mirror_v, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, (const int*)s.g_ko.as<int>(),
+                       (const int*)s.g_kg.as<int>(), (const int*)s.g_code.as<int>(), s.ng, s.g.len[0], s.g.len[1], s.g.len[2],
+                       s.spos.as<double4>(), s.svel.as<double4>());
+    TMD_HIP(hipGetLastError());
+}
+
 // REF_LAG refresh (diagnostic): the reference's pack from the immediate
 // source (which may be a not-yet-refreshed ghost) then unpack.
 __global__ void k_ghost_pack_lag(const double4* __restrict__ pos, const int* __restrict__ g_src, const int* __restrict__ g_code,
@@ -551,6 +583,20 @@ static void ensure_atom_capacity(Sim& s, int need) {
     s.inv_valid = 0;
     s.gsrc_valid = 0;
     s.cap_all = cap;
+}
+
+// End of the DEM mirror phase: the locals' positions and velocities back to
+// atom order, then the ghosts' from their owners.
+void dem_unmirror(Sim& s) {
+    if (has_error() || !s.dem_mirrored) return;
+    s.dem_mirrored = 0;
+    const int nt = 256;
+    if (s.nbinned > 0)
+        hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.spos.as<double4>(),
+                           (const int*)s.cell_parts.as<int>(), s.nbinned, s.n, s.pos.as<double4>(),
+                           (const double4*)s.svel.as<double4>(), s.vel.as<double4>());
+    TMD_HIP(hipGetLastError());
+    refresh_ghosts(s);  // also writes the ghosts' mirror entries, so the mirror stays that of pos
 }
 
 }  // namespace tmd

@@ -18,12 +18,13 @@ namespace tmd {
 // Cell-ordered position mirror: spos[k] = pos[parts[k]] (locals and ghosts);
 // with svel, also the locals' velocities in mirror order.
 __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict__ parts, const int* __restrict__ nbinned,
-                         double4* __restrict__ spos, int n, const double4* __restrict__ vel, double4* __restrict__ svel) {
+                         double4* __restrict__ spos, int n, const double4* __restrict__ vel, double4* __restrict__ svel,
+                         int vel_ghosts) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *nbinned) return;
     const int j = parts[k];
     spos[k] = pos[j];
-    if (svel && j < n) svel[k] = vel[j];
+    if (svel && (j < n || vel_ghosts)) svel[k] = vel[j];
 }
 
 // Per-atom inputs come in work order (mirror index work_k, cell work_c), so
@@ -152,6 +153,24 @@ void finish_windows(Sim& s) {
     apply_window_stats(s, s.h_winstats);
 }
 
+// DEM step loop: positions and velocities of every binned atom, ghosts
+// included, into the mirrors (spos, svel); the force pass reads them there.
+void dem_mirror(Sim& s) {
+    const int bound = s.n + s.ng;
+    if (has_error() || bound == 0) return;
+    s.svel.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+    s.svel2.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+    s.spos2.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+    if (has_error()) return;
+    const int nt = 256;
+    hipLaunchKernelGGL(k_mirror, dim3(div_up(bound, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
+                       (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
+                       s.spos.as<double4>(), s.n, (const double4*)s.vel.as<double4>(), s.svel.as<double4>(), 1);
+    TMD_HIP(hipGetLastError());
+    s.mirror_fresh = 1;
+    s.dem_mirrored = 1;
+}
+
 void refresh_mirror(Sim& s) {
     // the grid covers every atom that can be binned; the kernel stops at the
     // binned count on the device (the host's may still be pending)
@@ -167,7 +186,7 @@ void refresh_mirror(Sim& s) {
     hipLaunchKernelGGL(k_mirror, dim3(div_up(bound, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
                        (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
                        s.spos.as<double4>(), s.n, (const double4*)s.vel.as<double4>(),
-                       with_vel ? s.svel.as<double4>() : (double4*)nullptr);
+                       with_vel ? s.svel.as<double4>() : (double4*)nullptr, 0);
     TMD_HIP(hipGetLastError());
     s.mirror_fresh = 1;
     if (with_vel) s.vel_mirrored = 1;

@@ -363,6 +363,7 @@ int md_initialize_grid(double const* masses, struct Vector3D const* positions, s
     s.vel_mirrored = 0;
     s.vel_mirror_wanted = 0;
     s.ghost_pos_stale = 0;
+    s.dem_mirrored = 0;
     s.gsrc_valid = 0;
     s.h_mass.assign(masses, masses + nparticles);
     s.h_pos.resize(3 * (size_t)nparticles);
@@ -662,7 +663,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.sort_buf, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.g_ko, &s.g_kg, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.g_ko, &s.g_kg, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel, &s.svel2,
                       &s.nb_count, &s.nb_list, &s.nb_count2, &s.nb_list2,
                       &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
@@ -1090,9 +1091,39 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         }
         s.vel_mirror_wanted = 0;
         sync_pos_from_mirror(s);
+    } else if (dem && s.ghost_mode == GHOST_FRESH) {
+        // DEM: positions and velocities (ghosts included) live in the cell-
+        // ordered mirrors; the force pass ends each step and runs the next
+        // step's kick and drift (compute_dem_fused), and a step between
+        // rebuilds is the mirror-only ghost pass and one force pass
+        if (steps > 0) initial_integration(s, dt);
+        for (int j = 0; j < steps && !has_error(); ++j) {
+            hipEvent_t ev = nullptr;
+            if ((j + 1) % reneigh_every == 0) {
+                dem_unmirror(s);
+                rebuild(s, hf, cutoff + skin);
+            } else if (s.dem_mirrored) {
+                s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
+                dem_ghost_step(s);
+                s.timer.end(K_GHOST_SYNC, s.stream, ev);
+            } else {
+                s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
+                refresh_ghosts(s);
+                s.timer.end(K_GHOST_SYNC, s.stream, ev);
+            }
+            if (!s.dem_mirrored) {
+                s.timer.begin(K_MIRROR, s.stream, &ev);
+                dem_mirror(s);
+                s.timer.end(K_MIRROR, s.stream, ev);
+            }
+            compute_dem_fused(s, hf, cutoff, p0, p1, p2, p3, dt, j + 1 < steps);
+            ++done;
+        }
+        dem_unmirror(s);  // only after an error: the last step returns to atom order itself
     } else if (dem) {
-        // DEM: the step's final kick and the next step's kick and drift are
-        // one pass after the force
+        // DEM with the reference's lagging ghost refresh: atom order, the
+        // step's final kick and the next step's kick and drift one pass
+        // after the force
         if (steps > 0) initial_integration(s, dt);
         for (int j = 0; j < steps && !has_error(); ++j) {
             if ((j + 1) % reneigh_every == 0) {

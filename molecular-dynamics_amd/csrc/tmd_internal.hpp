@@ -143,6 +143,8 @@ struct Sim {
     DevBuf work_k;                     // mirror (CSR) index of each work slot's atom
     DevBuf work_c;                     // cell of each work slot's atom
     DevBuf svel;                       // local velocities in mirror order during the fused step loop
+    DevBuf svel2;                      // DEM step loop: the velocity mirror the force pass writes
+    int dem_mirrored = 0;              // DEM step loop: positions and velocities (ghosts too) live in the mirrors
     DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
     DevBuf g_ko, g_kg;                 // per ghost: mirror index of its owner and its own (-1: outside the grid)
     DevBuf nb_count, nb_list;
@@ -218,6 +220,13 @@ void build_neighbor_lists(Sim& s, int half, double cutoff);
 void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double kick_dt = 0.0, bool advance = false);
 // kick_dt > 0: the final kick of the step fused; advance: also the next
 // step's first kick and drift
+// DEM step loop with the state in the mirrors: mirror set-up, per-step ghost
+// pass, the force pass with the step's final kick and the next kick and
+// drift, and the way back to atom order
+void dem_mirror(Sim& s);
+void dem_ghost_step(Sim& s);
+void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double dt, bool advance);
+void dem_unmirror(Sim& s);
 void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, double mu, double kick_dt = 0.0,
                  bool advance = false);
 // eshift: subtract u(rc) from each in-cutoff pair's energy (plain-cutoff runs)

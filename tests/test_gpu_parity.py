@@ -595,3 +595,44 @@ def test_sort_particles_trajectory_bitwise(tm, loop):
     assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
     sim.close()
     orc.close()
+
+
+@pytest.mark.parametrize("half,precision", [(True, "fp64"), (True, "fp32mixed"), (False, "fp64")])
+def test_dem_step_loop_matches_driver_loop(tm, half, precision):
+    """The library's DEM step loop (state in the cell-ordered mirrors, kicks
+    and drift inside the force pass) against the same steps through the
+    md_* driver calls (main.cpp:715-762): bit-identical positions,
+    velocities and forces, rebuilds included."""
+    announce(f"DEM step loop vs md_* driver loop, half={half}, {precision}")
+    m, x, v = tm.generate_granular_gas(8, seed=11, vmax=1.0)
+    aabb = np.array([0.0, 8 * 1.6796, 0.0, 8 * 1.6796, 0.0, 8 * 1.6796]) * 0.6
+    x = x * 0.6  # compressed: spheres of diameter 1.0 in contact
+
+    def fresh():
+        s = tm.Simulation(m, x, v, aabb, 1.3, precision=precision, dem_radius=0.5)
+        s.copy_to_accelerator()
+        s.setup(half, 1.0, 0.3)
+        s.dem(half, 1.0, 0.5, 0.2, 100.0, 0.3)
+        return s
+    steps, ren, dt = 23, 10, 0.001
+    a = fresh()
+    a.run_dem(steps, ren, dt, half, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)
+    _, xa, va, fa = a.state()
+    a.close()
+    b = fresh()
+    for j in range(steps):
+        b.initial_integration(dt)
+        if (j + 1) % ren == 0:
+            b.exchange()
+            b.enforce_pbc()
+            b.borders()
+            b.distribute()
+            b.assemble_neighborlists(half, 1.3)
+        else:
+            b.synchronize_ghost_layer()
+        b.dem(half, 1.0, 0.5, 0.2, 100.0, 0.3)
+        b.final_integration(dt)
+    _, xb, vb, fb = b.state()
+    b.close()
+    assert np.abs(fa).max() > 1.0  # contacts
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(fa, fb)
