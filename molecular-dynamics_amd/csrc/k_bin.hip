@@ -238,8 +238,29 @@ TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz) {
 
 static void choose_tiles(Sim& s) {
     int t[3] = {s.tile_req[0], s.tile_req[1], s.tile_req[2]};
-    if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) t[0] = t[1] = t[2] = auto_tile_edge(s.g, s.n);
+    if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) {
+        if (s.tile_auto <= 0) s.tile_auto = auto_tile_edge(s.g, s.n);
+        t[0] = t[1] = t[2] = s.tile_auto;
+    }
     s.tiles = make_tile_geom(s.g, t[0], t[1], t[2]);
+}
+
+// auto_tile_edge sees only the mean density.  Where atoms crowd into part of
+// the box (the DEM gas fills half of it), the largest window can outgrow the
+// LDS share of the force pass in use (stage_bytes per slot within
+// stage_budget per block); the pass then reads such windows from global
+// memory.  From the peak window density of the current tiles this picks the
+// largest edge whose window fits, for the next binning.  The edge only
+// shrinks, and the tiles do not change the results (list order follows the
+// cells, not the tiles).
+void retune_tile_edge(Sim& s) {
+    if (s.tile_req[0] > 0 && s.tile_req[1] > 0 && s.tile_req[2] > 0) return;  // explicit tiles
+    const int T = s.tiles.tx;
+    if (T <= 1 || (double)(s.max_window + 1) * s.stage_bytes <= (double)s.stage_budget) return;
+    const double peak = (s.max_window + 1) / ((T + 2.0) * (T + 2.0) * (T + 2.0));
+    int t = T - 1;
+    while (t > 1 && (t + 2.0) * (t + 2.0) * (t + 2.0) * peak * s.stage_bytes > (double)s.stage_budget) --t;
+    s.tile_auto = std::min(t, s.tile_auto > 0 ? s.tile_auto : t);
 }
 
 // Counting sort of atoms 0..nall-1 by cell: cell_count, cell_start (CSR),

@@ -325,8 +325,6 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS,
 // velocity and the position in atom order.  The pair arithmetic, the
 // contact decisions and the summation order are k_dem's, and the kicks are
 // k_dem_finish's, so the trajectory is that of the separate passes.
-constexpr int kDemStageBytes = 48;
-
 static int dem_window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kDemStageBytes); }
 
 template <int PREC, bool PAIR2, int ADV>

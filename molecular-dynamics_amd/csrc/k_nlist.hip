@@ -91,6 +91,7 @@ static void apply_window_stats(Sim& s, const int* host) {
     s.nbinned = host[1];
     s.max_window = host[2];
     s.tile_threads = tile_block_threads(s.tile_threads_req, host[3]);
+    retune_tile_edge(s);
     s.spos.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
     s.spos2.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
     if (has_error()) return;

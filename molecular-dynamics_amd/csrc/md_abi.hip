@@ -676,6 +676,9 @@ void md_deallocate_grid() {
     s.windows_pending = 0;
     s.defer_window_sync = 0;
     s.max_window = 0;
+    s.tile_auto = 0;
+    s.stage_bytes = kStageBytes;
+    s.stage_budget = kLdsBudget;
     s.n = s.ng = 0;
     s.cap_all = 0;
     s.nb_valid = 0;
@@ -896,6 +899,7 @@ void tmd_set_tile(int tx, int ty, int tz) {
     s.tile_req[0] = tx;
     s.tile_req[1] = ty;
     s.tile_req[2] = tz;
+    s.tile_auto = 0;
     s.windows_valid = 0;
     s.nb_valid = 0;
 }
@@ -1095,7 +1099,13 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         // DEM: positions and velocities (ghosts included) live in the cell-
         // ordered mirrors; the force pass ends each step and runs the next
         // step's kick and drift (compute_dem_fused), and a step between
-        // rebuilds is the mirror-only ghost pass and one force pass
+        // rebuilds is the mirror-only ghost pass and one force pass.  That
+        // pass stages positions and velocities (48 B per window slot), so
+        // automatic tiles are sized for three of its blocks per CU from the
+        // next rebuild on.
+        s.stage_bytes = kDemStageBytes;
+        s.stage_budget = kDemLdsShare;
+        retune_tile_edge(s);
         if (steps > 0) initial_integration(s, dt);
         for (int j = 0; j < steps && !has_error(); ++j) {
             hipEvent_t ev = nullptr;

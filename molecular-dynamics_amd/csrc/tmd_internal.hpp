@@ -97,6 +97,9 @@ struct Sim {
     int max_window = 0; // largest tile window (atoms)
     TileGeom tiles{};   // tile decomposition of the cell grid
     int tile_req[3] = {0, 0, 0};  // requested cells per tile (0: automatic)
+    int tile_auto = 0;            // automatic cells per tile edge in effect (0: chosen at the next binning)
+    int stage_bytes = kStageBytes;  // LDS bytes per window slot of the force pass in use
+    int stage_budget = kLdsBudget;  // LDS bytes per block that pass should stay within
     int tile_threads = kTileThreads;  // threads per tile block in use (multiple of 64, <= kMaxTileThreads)
     int tile_threads_req = 0;         // requested block size, 0: automatic (tile_block_threads)
     int windows_valid = 0;
@@ -203,6 +206,7 @@ void make_geom(Geom& G, const double* aabb, const double* rank_aabb, double cell
 void bin_particles(Sim& s);
 void sort_locals(Sim& s);  // PBC wrap + renumber the locals in cell order
 int auto_tile_edge(const Geom& G, int n);  // cells per tile edge for ~480 locals per tile
+void retune_tile_edge(Sim& s);  // automatic tiles: a smaller edge at the next binning when windows overflow
 TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz);
 
 // tile windows (after binning), position mirror, neighbour lists
