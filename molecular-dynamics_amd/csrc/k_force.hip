@@ -215,22 +215,41 @@ struct DEMArgs {
 // before any rounding to R; FP32 turns it into delta = ((2R)^2 - r^2) /
 // (2R + r), since 2R - r in FP32 would lose a small overlap's leading
 // digits.
+//
+// FP64 divides and takes square roots as written.  FP32 works from the
+// hardware reciprocal square root and reciprocal (v_rsq_f32, v_rcp_f32, about
+// 1 ulp each): 1/r once, then products, instead of four IEEE divisions and two
+// square roots; the rounding this adds is a few 1e-8 relative, far inside
+// the FP32-mixed tolerance, and it halves the contact term's instructions.
 template <typename R>
 __device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R rsq, R dd, R vx, R vy, R vz, R& Fx, R& Fy,
                                          R& Fz) {
     const R d2 = (R)a.d2;
-    const R r = sqrt(rsq);
-    // FP64 keeps the reference's 2R - r
-    R delta;
-    if constexpr (sizeof(R) == sizeof(double)) delta = d2 - r;
-    else delta = dd / (d2 + r);
-    const R nx = dx / r, ny = dy / r, nz = dz / r;
+    R delta, nx, ny, nz;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const R r = sqrt(rsq);
+        delta = d2 - r;  // the reference's 2R - r
+        nx = dx / r; ny = dy / r; nz = dz / r;
+    } else {
+        const float ir = __builtin_amdgcn_rsqf(rsq);  // contact: rsq in (0, (2R)^2)
+        delta = dd * __builtin_amdgcn_rcpf(d2 + rsq * ir);
+        nx = dx * ir; ny = dy * ir; nz = dz * ir;
+    }
     const R vn = vx * nx + vy * ny + vz * nz;
     const R tx = vx - vn * nx, ty = vy - vn * ny, tz = vz - vn * nz;
-    const R vtl = sqrt(tx * tx + ty * ty + tz * tz);
+    const R vt2 = tx * tx + ty * ty + tz * tz;
     const R fn = (R)a.k * delta - (R)a.gn * vn;
-    const R ft = fmin((R)a.gt * vtl, (R)a.mu * fabs(fn));
-    const R inv = vtl > R(0) ? ft / vtl : R(0);
+    R inv;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const R vtl = sqrt(vt2);
+        const R ft = fmin((R)a.gt * vtl, (R)a.mu * fabs(fn));
+        inv = vtl > R(0) ? ft / vtl : R(0);
+    } else {
+        // |v_t| = vt2 / sqrt(vt2); below 1e-30 (and at 0) the tangential term is 0
+        const float ivt = vt2 > 1e-30f ? __builtin_amdgcn_rsqf(vt2) : 0.0f;
+        const float ft = fminf((float)a.gt * (vt2 * ivt), (float)a.mu * fabsf(fn));
+        inv = ft * ivt;
+    }
     Fx = fn * nx - inv * tx;
     Fy = fn * ny - inv * ty;
     Fz = fn * nz - inv * tz;
