@@ -528,7 +528,12 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_lj_full(const RepDesc* __
         const int i = work[w];
         const int kw = wk[w];
         const double4 pi = spos[kw];
-        const int cnt = nbc[w];
+        // A captured run goes on after a list build that overflowed (the
+        // host sees the flag at the end, restores the start and reruns with
+        // a larger capacity), so a count can exceed the capacity here: read
+        // at most `cap` entries, which keeps every list load inside the
+        // atom's column; that run's results are discarded.
+        const int cnt = min(nbc[w], a.cap);
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
         const double4 f = staged ? lj_full_atom<PREC, SHIFT, UNIT, true>(pi, cnt, lp, sp, spos, W, a)
                                  : lj_full_atom<PREC, SHIFT, UNIT, false>(pi, cnt, lp, sp, spos, W, a);
@@ -580,8 +585,9 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_energy(const RepDesc* __r
     for (int at = threadIdx.x; at < nloc; at += blockDim.x) {
         const int w = w0 + at;
         const int4* lp = reinterpret_cast<const int4*>(nbl + (size_t)(w >> 6) * (size_t)a.cap * 64) + (w & 63);
-        if (staged) lj_energy_atom<SHIFT, true>(pos[work[w]], nbc[w], lp, sp, spos, parts, W, a, 0, D.a0 + D.n, pe, vir, np);
-        else lj_energy_atom<SHIFT, false>(pos[work[w]], nbc[w], lp, sp, spos, parts, W, a, 0, D.a0 + D.n, pe, vir, np);
+        const int cnt = min(nbc[w], a.cap);  // as in k_b_lj_full
+        if (staged) lj_energy_atom<SHIFT, true>(pos[work[w]], cnt, lp, sp, spos, parts, W, a, 0, D.a0 + D.n, pe, vir, np);
+        else lj_energy_atom<SHIFT, false>(pos[work[w]], cnt, lp, sp, spos, parts, W, a, 0, D.a0 + D.n, pe, vir, np);
     }
     const double bpe = block_sum_double_any(pe);
     const double bvir = block_sum_double_any(vir);

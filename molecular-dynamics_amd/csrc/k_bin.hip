@@ -213,11 +213,15 @@ __global__ void __launch_bounds__(256) k_tile_fill_block(const int* __restrict__
 }
 
 // Cells per tile: about 480 locals per tile (one 512-thread block), with
-// the window (tile + 1 cell on each side) inside the LDS budget.  An
-// explicit choice (tmd_set_tile) overrides.
+// the window (tile + 1 cell on each side) inside the LDS budget.  The
+// density is taken over the domain's volume in cells, not over the grid's
+// inner cells: the grid rounds each side up by up to a cell and a half, so
+// a small box would look up to twice as sparse as it is.  An explicit choice
+// (tmd_set_tile) overrides.
 int auto_tile_edge(const Geom& G, int n) {
-    const double inner = (double)std::max(G.nx - 2, 1) * std::max(G.ny - 2, 1) * std::max(G.nz - 2, 1);
-    const double rho = std::max((double)n / inner, 1e-3);
+    double cells = 1.0;
+    for (int d = 0; d < 3; ++d) cells *= std::max((G.rank[2 * d + 1] - G.rank[2 * d]) / G.spacing, 1.0);
+    const double rho = std::max((double)n / cells, 1e-3);
     int T = (int)std::lround(std::cbrt(480.0 / rho));
     T = std::max(1, std::min(T, kMaxTile));
     while (T > 1 && (T + 2.0) * (T + 2.0) * (T + 2.0) * rho * 28.0 > 0.85 * kLdsBudget) --T;
