@@ -112,6 +112,10 @@ int tmd_load_state(const char* path);
 /* ---- introspection for parity tests (host copies) --------------------------- */
 /* {nx, ny, nz, ncells, nlocal, nghost, list capacity} */
 void tmd_get_grid_info(int* out7);
+/* Tile decomposition of the last binning: {cells per tile in x, y, z,
+ * tiles, threads per tile block, largest window (atoms), LDS bytes per
+ * window slot that automatic tiles are sized for}. */
+void tmd_get_tile_info(int* out7);
 /* Ghost g (slot nlocal + g): flattened owner, total image (3 ints),
  * immediate source and per-direction flag (3 ints).  Returns nghost. */
 int tmd_get_ghost_table(int* owner, int* image3, int* source, int* flag3);

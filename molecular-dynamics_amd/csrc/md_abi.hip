@@ -977,6 +977,13 @@ void tmd_get_grid_info(int* out) {
     out[4] = s.n; out[5] = s.ng; out[6] = s.nb_cap;
 }
 
+void tmd_get_tile_info(int* out) {
+    const Sim& s = sim();
+    out[0] = s.tiles.tx; out[1] = s.tiles.ty; out[2] = s.tiles.tz;
+    out[3] = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
+    out[4] = s.tile_threads; out[5] = s.max_window; out[6] = s.stage_bytes;
+}
+
 int tmd_get_ghost_table(int* owner, int* image3, int* source, int* flag3) {
     Sim& s = sim();
     if (!s.stream || has_error() || s.ng == 0) return s.ng;

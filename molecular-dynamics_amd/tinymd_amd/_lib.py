@@ -90,6 +90,7 @@ SIGNATURES = {
     "tmd_load_state": (_i, [ctypes.c_char_p]),
     "tmd_thermo_noalloc": (None, [_d, _d, _d, _dp]),
     "tmd_get_grid_info": (None, [_ip]),
+    "tmd_get_tile_info": (None, [_ip]),
     "tmd_get_ghost_table": (_i, [_ip, _ip, _ip, _ip]),
     "tmd_get_cells": (_i, [_ip, _ip, _ip]),
     "tmd_get_neighbor_lists": (_i, [_ip, _ip, _i]),

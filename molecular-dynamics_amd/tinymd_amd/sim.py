@@ -70,6 +70,10 @@ class Simulation:
         L.md_deallocate_grid()
         self.set_modes(precision, half_mode, ghost_mode, lj_shift, dem_radius, sync)
         L.tmd_set_sort_particles(int(bool(sort)))
+        # tile shape and block size are library-wide settings: a new
+        # simulation starts from the automatic ones (set_tile after this)
+        L.tmd_set_tile(0, 0, 0)
+        L.tmd_set_tile_threads(0)
         m = np.ascontiguousarray(masses, dtype=np.float64)
         x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         v = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
@@ -122,6 +126,8 @@ class Simulation:
         L.tmd_set_sync(int(bool(sync)))
         L.tmd_set_adjoint_precision(-1)
         L.tmd_set_sort_particles(0)
+        L.tmd_set_tile(0, 0, 0)
+        L.tmd_set_tile_threads(0)
         self.n = L.tmd_load_state(os.fsencode(path))
         check()
         self.aabb = None
@@ -285,6 +291,11 @@ class Simulation:
         out = np.zeros(7, dtype=np.int32)
         self.L.tmd_get_grid_info(_iptr(out))
         return dict(zip(["nx", "ny", "nz", "ncells", "n", "ng", "nb_cap"], [int(a) for a in out]))
+
+    def tile_info(self):
+        out = np.zeros(7, dtype=np.int32)
+        self.L.tmd_get_tile_info(_iptr(out))
+        return dict(zip(["tx", "ty", "tz", "ntiles", "threads", "max_window", "stage_bytes"], [int(a) for a in out]))
 
     def ghost_table(self):
         ng = self.nghost

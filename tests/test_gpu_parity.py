@@ -636,3 +636,40 @@ def test_dem_step_loop_matches_driver_loop(tm, half, precision):
     b.close()
     assert np.abs(fa).max() > 1.0  # contacts
     assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(fa, fb)
+
+
+def test_dem_tile_edge_retune_keeps_trajectory(tm):
+    """A granular gas that fills half of its box: the mean density gives
+    8-cell automatic tiles whose densest windows overflow the DEM step
+    pass's LDS share (48 B per slot, three blocks per CU); the step loop
+    shrinks the edge at its first rebuild.  The trajectory is bit-identical
+    to runs with explicit 6- and 8-cell tiles (the lists follow the cells,
+    not the tiles)."""
+    announce("DEM automatic tile edge retune, 16^3 gas, 45 steps")
+    m, x, v = tm.generate_granular_gas(16)
+    L = 16 * 1.6796
+    aabb = np.array([0.0, L, 0.0, L, 0.0, L])
+
+    def run(tile):
+        s = tm.Simulation(m, x, v, aabb, 1.3, precision="fp32mixed", dem_radius=0.5)
+        if tile:
+            s.set_tile(tile)
+        s.copy_to_accelerator()
+        s.setup(True, 1.0, 0.3)
+        s.dem(True, 1.0, 0.5, 0.2, 100.0, 0.3)
+        before = s.tile_info()
+        s.run_dem(45, 20, 0.001, True, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)
+        after = s.tile_info()
+        _, xs, vs, fs = s.state()
+        s.close()
+        return before, after, (xs, vs, fs)
+
+    b0, a0, r0 = run(0)
+    announce(f"  automatic tiles: {b0['tx']} -> {a0['tx']} cells, largest window {b0['max_window']} -> {a0['max_window']}")
+    assert b0["tx"] == 8 and (b0["max_window"] + 1) * 48 > 160 * 1024 // 3 - 1024
+    assert a0["tx"] < 8 and (a0["max_window"] + 1) * a0["stage_bytes"] <= 160 * 1024 // 3 - 1024
+    for t in (6, 8):
+        _, a, r = run(t)
+        assert a["tx"] == t
+        for p, q in zip(r0, r):
+            assert np.array_equal(p, q), f"tile edge {t}"
