@@ -513,6 +513,7 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
     TMD_REQUIRE(half == s.nb_half, "compute forces: half flag differs from the one the neighbour list was built with");
     if (has_error() || s.n == 0) return;
     LJArgs a = make_lj_args(s, rc, eps, sigma);
+    a.wcap = window_cap_pos(s);  // the force pass stages positions only
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     const int shift = s.lj_shift;
     s.last_force_rc = rc;
@@ -523,7 +524,7 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
     s.timer.begin(K_FORCE, s.stream, &ev);
     {
         double4* F = s.force.as<double4>();
-        const size_t lds = lds_bytes(s, false);
+        const size_t lds = (size_t)a.wcap * kPosStageBytes;
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
         const double hdt = 0.5 * kick_dt;
         // inside the fused step loop the velocities live in mirror order

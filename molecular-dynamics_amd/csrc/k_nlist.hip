@@ -38,7 +38,9 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __rest
                                                         int* __restrict__ nb_count2, nb_t* __restrict__ nb_list2,
                                                         int* __restrict__ overflow) {
     extern __shared__ double dyn[];
-    int* cst = reinterpret_cast<int*>(reinterpret_cast<char*>(dyn) + (size_t)kStageBytes * a.wcap);
+    // half lists stage the atom index too (the i < j test), full lists positions only
+    constexpr int SB = HALF ? kStageBytes : kPosStageBytes;
+    int* cst = reinterpret_cast<int*>(reinterpret_cast<char*>(dyn) + (size_t)SB * a.wcap);
     const int tile = blockIdx.x;
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
@@ -75,6 +77,7 @@ int tile_block_threads(int requested, int max_locals) {
 }
 
 int window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kStageBytes); }  // + sentinel slot
+int window_cap_pos(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kPosStageBytes); }
 
 // The window statistics (locals placed, binned atoms, largest window, most
 // locals in a tile) decide the LDS size and block size of the tile kernels.
@@ -215,9 +218,12 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         if (has_error()) return;
         TMD_HIP(hipMemsetAsync(counters + 2, 0, sizeof(int), s.stream));
         const int rsb = row_starts_bytes(s.tiles);
-        const int wcap = std::min(s.max_window + kListPad, (kLdsBudget - rsb) / kStageBytes);  // + sentinel slots
+        // window slots (positions, 24 B; half lists also the atom index,
+        // 28 B) + sentinel slots, then the row starts
+        const int sb = half ? kStageBytes : kPosStageBytes;
+        const int wcap = std::min(s.max_window + kListPad, (kLdsBudget - rsb) / sb);
         NbArgs a{s.n, s.nb_cap, half, s.half_mode, wcap, cutoff * cutoff, s.tiles};
-        const size_t lds = (size_t)wcap * kStageBytes + rsb;
+        const size_t lds = (size_t)wcap * sb + rsb;
         if (s.n > 0) {
 #define TMD_NLIST(H)                                                                                                   \
     hipLaunchKernelGGL(k_nlist<H>, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(), \

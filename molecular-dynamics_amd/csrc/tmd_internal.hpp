@@ -211,7 +211,8 @@ TileGeom make_tile_geom(const Geom& G, int tx, int ty, int tz);
 
 // tile windows (after binning), position mirror, neighbour lists
 void build_windows(Sim& s);
-int window_cap(const Sim& s);
+int window_cap(const Sim& s);      // window slots of kernels that stage positions and atom indices
+int window_cap_pos(const Sim& s);  // window slots of kernels that stage positions only
 int tile_block_threads(int requested, int max_locals);  // staged window atoms per block (LDS budget, kStageBytes each)
 void refresh_mirror(Sim& s);
 void finish_windows(Sim& s);  // apply window statistics left pending by build_windows

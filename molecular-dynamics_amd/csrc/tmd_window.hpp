@@ -32,6 +32,7 @@ constexpr int kWinMax = 65535;     // window offsets are 16-bit
 // block's static LDS (window rows, reductions)
 constexpr int kLdsBudget = 80 * 1024 - 1024;
 constexpr int kStageBytes = 28;       // LDS bytes per staged window atom: x, y, z 24, index 4
+constexpr int kPosStageBytes = 24;    // the same without the index (kernels that stage positions only)
 constexpr int kDemStageBytes = 48;    // DEM step pass: position and velocity, 24 + 24
 // LDS share per block of the DEM step pass: three blocks per CU (160 KiB of
 // LDS) less the static tile table
