@@ -677,7 +677,7 @@ void md_deallocate_grid() {
     s.defer_window_sync = 0;
     s.max_window = 0;
     s.tile_auto = 0;
-    s.stage_bytes = kStageBytes;
+    s.stage_bytes = kPosStageBytes;
     s.stage_budget = kLdsBudget;
     s.n = s.ng = 0;
     s.cap_all = 0;
@@ -1084,7 +1084,10 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
     if (!dem && s.ghost_mode == GHOST_FRESH) {
         // LJ: the force pass also performs the final kick of its step and
         // the first kick + drift of the next one, so a step is a ghost/mirror
-        // refresh (or a rebuild) and one force pass
+        // refresh (or a rebuild) and one force pass.  Automatic tiles are
+        // sized for that pass's positions-only window slots.
+        s.stage_bytes = kPosStageBytes;
+        s.stage_budget = kLdsBudget;
         if (steps > 0) initial_integration(s, dt);
         mirror_velocities(s, true);
         s.vel_mirror_wanted = 1;  // the rebuild's mirror pass moves them again

@@ -98,7 +98,7 @@ struct Sim {
     TileGeom tiles{};   // tile decomposition of the cell grid
     int tile_req[3] = {0, 0, 0};  // requested cells per tile (0: automatic)
     int tile_auto = 0;            // automatic cells per tile edge in effect (0: chosen at the next binning)
-    int stage_bytes = kStageBytes;  // LDS bytes per window slot of the force pass in use
+    int stage_bytes = kPosStageBytes;  // LDS bytes per window slot of the force pass in use (LJ: positions)
     int stage_budget = kLdsBudget;  // LDS bytes per block that pass should stay within
     int tile_threads = kTileThreads;  // threads per tile block in use (multiple of 64, <= kMaxTileThreads)
     int tile_threads_req = 0;         // requested block size, 0: automatic (tile_block_threads)
