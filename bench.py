@@ -28,6 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+XGMI_LINK_GBS = 153.0     # one MI355X xGMI link, per direction (GB/s)
 FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
 METRIC = "LJ atom-steps/s, FP32-mixed vs FP64 at ≤1e-5 force err; adjoint/fwd cost"
@@ -72,13 +73,13 @@ PHASES = {"force": ("force", "integrate1", "integrate2"), "neigh": ("neighbor",)
           "comm": ("borders", "ghost_sync"), "other": ("bin", "mirror")}
 
 
-def run_variant(tm, precision, half, steps, warmup, profile=False):
+def run_variant(tm, precision, half, steps, warmup, profile=False, nc=None):
     """Timed run of `steps` steps (no per-kernel timers), then, with
     profile, a second run of the same length with HIP-event timers on the
     library stream for per-kernel averages and the reference's phase split.
     Energy drift is taken with the energy-shifted potential U(r) - U(rc)
     (continuous at the cutoff) plus the kinetic energy."""
-    sim = tm.Simulation.fcc(NC, precision=precision)
+    sim = tm.Simulation.fcc(nc or NC, precision=precision)
     sim.setup(half, RC, SKIN)
     sim.lennard_jones(half, RC)
     if warmup:
@@ -359,6 +360,27 @@ def dem_leg(tm, steps=1000, nc=64):
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
+def shard_leg(tm, steps=200, warmup=20, nc=40):
+    """One GPU's share of the 80^3 box split 2 x 2 x 2 over eight GPUs: a
+    40^3-cell sub-box (256,000 atoms) with its ghost shell.  A periodic
+    40^3 box has the same locals, the same ghost-shell population and the
+    same per-step work; only the source of the ghosts' positions differs
+    (owners on this GPU here, face neighbours over xGMI there).  Reports the
+    step rate, the force pass, the ghost refresh (the pack and unpack of the
+    halo) and the bytes a face exchange would move per step."""
+    sim, r = run_variant(tm, "fp32mixed", False, steps, warmup, profile=True, nc=nc)
+    n, ng = sim.nlocal, sim.nghost
+    sim.close()
+    halo = 24 * ng  # FP64 positions of the ghost shell, per step
+    return {"cells": f"{nc}^3", "atoms": n, "ghosts_last_rebuild": ng, "steps": steps,
+            "atom_steps_per_s": r["atom_steps_per_s"], "ms_per_step": r["seconds"] / steps * 1e3,
+            "force_ms_avg": r["force_ms_avg"], "ghost_refresh_ms_avg": r["ghost_sync_ms_avg"],
+            "phases_ms_per_step": r["phases_ms_per_step"], "halo_bytes_per_step": halo,
+            "halo_us_one_xgmi_link": halo / XGMI_LINK_GBS / 1e3,
+            "halo_us_three_links": halo / 3 / XGMI_LINK_GBS / 1e3,
+            "xgmi_link_GBps": XGMI_LINK_GBS}
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size (torch), for scaling the
     FETCH_SIZE / WRITE_SIZE counters of the force kernel."""
@@ -443,6 +465,7 @@ def main():
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--copy-bw", action="store_true", help="only measure the device copy bandwidth and exit")
     ap.add_argument("--no-copy-bw", action="store_true", help="skip the device copy bandwidth leg")
+    ap.add_argument("--no-shard", action="store_true", help="skip the one-GPU-share (40^3 sub-box) leg")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -503,6 +526,11 @@ def main():
         dem = dem_leg(tm, steps=200 if args.quick else 1000)
         print(f"# dem: {dem}", file=sys.stderr, flush=True)
 
+    shard = None
+    if not args.no_shard:
+        shard = shard_leg(tm)
+        print(f"# shard 40^3: {shard}", file=sys.stderr, flush=True)
+
     value = r32["atom_steps_per_s"]
     t_force = r32["force_ms_avg"] * 1e-3
     achieved = model["bytes"] / t_force / 1e9
@@ -562,6 +590,8 @@ def main():
         line["batched"] = batched
     if dem:
         line["dem"] = dem
+    if shard:
+        line["shard_40cube_of_80cube_2x2x2"] = shard
     if "fp64_full" in variants:
         line["fp32mixed_over_fp64"] = value / variants["fp64_full"]["atom_steps_per_s"]
     if not args.no_cpu:
