@@ -673,3 +673,19 @@ def test_dem_tile_edge_retune_keeps_trajectory(tm):
         assert a["tx"] == t
         for p, q in zip(r0, r):
             assert np.array_equal(p, q), f"tile edge {t}"
+
+
+def test_non_finite_position_fails_loudly(tm):
+    """A local atom with a non-finite coordinate falls outside every cell.
+    The reference's fill_cells skips such an atom silently
+    (core/grid.impala:296-311); the library stops with an error at the
+    binning instead of running on without it."""
+    announce("non-finite position -> error at binning")
+    aabb, m, x, v = perturbed_fcc(4, amp=0.05, seed=3)
+    x = x.copy()
+    x[5, 1] = np.nan
+    sim = tm.Simulation(m, x, v, aabb, RC + SKIN)
+    sim.copy_to_accelerator()
+    with pytest.raises(tm.TinyMDError, match="outside"):
+        sim.setup(False, RC, SKIN)
+    sim.close()
