@@ -689,3 +689,36 @@ def test_non_finite_position_fails_loudly(tm):
     with pytest.raises(tm.TinyMDError, match="outside"):
         sim.setup(False, RC, SKIN)
     sim.close()
+
+
+def test_dem_step_loop_unstaged_windows(tm):
+    """The DEM step pass with a window beyond its LDS budget (one 8-cell tile
+    over a compressed gas: every neighbour read from the mirrors) against
+    the same loop with staged windows: bit-identical positions, velocities
+    and forces, rebuilds included."""
+    announce("DEM step loop: unstaged (8-cell tile) vs staged windows")
+    m, x, v = tm.generate_granular_gas(8, seed=11, vmax=1.0)
+    aabb = np.array([0.0, 8 * 1.6796, 0.0, 8 * 1.6796, 0.0, 8 * 1.6796]) * 0.6
+    x = x * 0.6
+
+    def run(tile):
+        s = tm.Simulation(m, x, v, aabb, 1.3, precision="fp32mixed", dem_radius=0.5)
+        if tile:
+            s.set_tile(tile)
+        s.copy_to_accelerator()
+        s.setup(True, 1.0, 0.3)
+        s.dem(True, 1.0, 0.5, 0.2, 100.0, 0.3)
+        s.run_dem(23, 10, 0.001, True, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)
+        info = s.tile_info()
+        _, xs, vs, fs = s.state()
+        s.close()
+        return info, (xs, vs, fs)
+
+    i_st, r_st = run(0)
+    i_un, r_un = run(8)
+    kLdsBudget = 80 * 1024 - 1024
+    assert (i_st["max_window"] + 1) * 48 <= kLdsBudget
+    assert (i_un["max_window"] + 1) * 48 > kLdsBudget
+    assert np.abs(r_st[2]).max() > 1.0  # contacts
+    for p, q in zip(r_st, r_un):
+        assert np.array_equal(p, q)
