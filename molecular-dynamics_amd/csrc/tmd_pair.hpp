@@ -95,30 +95,42 @@ __device__ __forceinline__ void win_pos(int t, const WinStage& st, const double4
 // past the count in the last group are the sentinel offset (beyond the
 // cutoff), so only the cutoff test masks; from the mirror, the count does.
 // FP32 variants form rsq with FMAs; FP64 keeps the reference's rounding.
-template <int PREC, int SHIFT, int UNIT, bool STAGED>
+//
+// PAIR2 (half lists): the lane's groups are column 1's (cnt entries at lp)
+// followed by column 2's (cnt2 at lp2), one running sum over both; a
+// wavefront then runs as many groups as its busiest lane has in both
+// columns together, not the busiest lane of each column in turn.
+template <int PREC, int SHIFT, int UNIT, bool STAGED, bool PAIR2 = false>
 __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const WinStage& sp,
-                                                const double4* __restrict__ spos, const TileWin& W, const LJArgs& a) {
+                                                const double4* __restrict__ spos, const TileWin& W, const LJArgs& a,
+                                                int cnt2 = 0, const int4* __restrict__ lp2 = nullptr) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
-    const int ng = (cnt + 7) >> 3;
+    const int ng1 = (cnt + 7) >> 3;
+    const int ng = ng1 + (PAIR2 ? (cnt2 + 7) >> 3 : 0);
+    // group g of the lane's entries (group 0 of either column always
+    // exists: capacity >= 8)
+    auto grp = [&](int g) -> const int4* {
+        return (!PAIR2 || g < ng1) ? lp + (size_t)g * 64 : lp2 + (size_t)(g - ng1) * 64;
+    };
     // the next group of 8 entries is loaded while the current one is used
     // (streaming loads: each group is read once per pass)
-    int4 q = ld_stream(lp);  // group 0 of the column always exists (capacity >= 8)
+    int4 q = ld_stream(grp(0));
     for (int gi = 0; gi < ng; ++gi) {
         const int4 e = q;
-        q = ld_stream(lp + (size_t)min(gi + 1, ng - 1) * 64);  // next group (clamped: always a valid load)
+        q = ld_stream(grp(min(gi + 1, ng - 1)));  // next group (clamped: always a valid load)
+        const int rem = (!PAIR2 || gi < ng1) ? cnt - gi * 8 : cnt2 - (gi - ng1) * 8;  // entries in this group
         // two passes of 4 entries (fewer live registers than 8 at once),
         // unrolled: the compiler then overlaps one pass's LDS reads with the
         // other's arithmetic
 #pragma unroll
         for (int hb = 0; hb < 8; hb += 4) {
-        const int k0 = gi * 8 + hb;
-        if (k0 >= cnt) break;  // this lane's entries are done
+        if (hb >= rem) break;  // this lane's entries are done
         double dx[4], dy[4], dz[4], rs[4];
         bool ok[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const bool valid = STAGED || k0 + u < cnt;
+            const bool valid = STAGED || hb + u < rem;
             double xj, yj, zj;
             win_pos<STAGED>(valid ? nb_entry(e, hb + u) : 0, sp, spos, W, xj, yj, zj);
             dx[u] = pi.x - xj;
