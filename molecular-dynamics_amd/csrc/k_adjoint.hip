@@ -36,7 +36,7 @@ struct AdjCoef {
 };
 
 template <int PREC, int SHIFT>
-__global__ void TMD_TILE_KERNEL k_lj_adjoint(const double4* __restrict__ pos, const double4* __restrict__ spos,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_adjoint(const double4* __restrict__ pos, const double4* __restrict__ spos,
                                                              const int* __restrict__ start, const int* __restrict__ parts,
                                                              const int* __restrict__ work, const int* __restrict__ tile_start,
                                                              const int* __restrict__ nbc, const nb_t* __restrict__ nbl,

@@ -466,7 +466,7 @@ __global__ void k_b_mirror(const double4* __restrict__ pos, const int* __restric
 
 // ---------------------------------------------------------------------------
 // neighbour lists and forces: one block per tile of any replica
-__global__ void TMD_TILE_KERNEL k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+__global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                           const double4* __restrict__ spos, const int* __restrict__ work_k,
                                                           const int* __restrict__ cell_of,
                                                           const int* __restrict__ start,
@@ -507,7 +507,7 @@ __global__ void TMD_TILE_KERNEL k_b_nlist(const RepDesc* __restrict__ R, const i
 // step's first kick and drift, written only into the second mirror, with no
 // force store (as k_lj_full<..., 2>); ADV 0 stores the force.
 template <int PREC, int SHIFT, int UNIT, int ADV>
-__global__ void TMD_TILE_KERNEL k_b_lj_full(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+__global__ void __launch_bounds__(kMaxTileThreads) k_b_lj_full(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                                const double4* __restrict__ spos, const int* __restrict__ start,
                                                                const int* __restrict__ work, const int* __restrict__ wk,
                                                                const int* __restrict__ tile_start, const int* __restrict__ nbc,
@@ -559,7 +559,7 @@ __global__ void TMD_TILE_KERNEL k_b_lj_full(const RepDesc* __restrict__ R, const
 }
 
 template <int SHIFT>
-__global__ void TMD_TILE_KERNEL k_b_energy(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
+__global__ void __launch_bounds__(kMaxTileThreads) k_b_energy(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
                                                            const double4* __restrict__ pos, const double4* __restrict__ spos,
                                                            const int* __restrict__ start, const int* __restrict__ parts,
                                                            const int* __restrict__ work, const int* __restrict__ tile_start,

@@ -57,7 +57,7 @@ namespace tmd {
 // negated reaction the reference scatters to i (dx and dv change sign
 // exactly), so only the summation order differs from the reference's.
 template <int PREC, int SHIFT, int UNIT, int KICK, bool PAIR2>
-__global__ void TMD_TILE_KERNEL k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
                                                          double4* __restrict__ vel, double hdt, double dt,
                                                          const int* __restrict__ wk, double4* __restrict__ spos_next,
                                                          int vel_by_k, const int* __restrict__ nbc2,
@@ -143,7 +143,7 @@ __global__ void k_dem_finish(int n, const double4* __restrict__ force, double4* 
 // local-local pairs 1 and local-ghost pairs 1/2.  Per-block partial sums in
 // a fixed order, reduced by one block in a fixed order: deterministic.
 template <int SHIFT>
-__global__ void TMD_TILE_KERNEL k_lj_energy(TMD_KERNEL_LIST_PARAMS, LJArgs a, int half,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARAMS, LJArgs a, int half,
                                                            double* __restrict__ partials) {
     extern __shared__ double dyn[];
     const int tile = blockIdx.x;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void dem_pair(const DEMArgs& a, R dx, R dy, R dz, R r
 // bit, the negated reaction the half list would send to i (dx and dv change
 // sign exactly), so no scatter is needed.
 template <int PREC, bool PAIR2>
-__global__ void TMD_TILE_KERNEL k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __restrict__ vel, DEMArgs a,
+__global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __restrict__ vel, DEMArgs a,
                                                      double4* __restrict__ force, const int* __restrict__ nbc2,
                                                      const nb_t* __restrict__ nbl2) {
     extern __shared__ double dyn[];
@@ -343,7 +343,7 @@ __global__ void TMD_TILE_KERNEL k_dem(TMD_KERNEL_LIST_PARAMS, const double4* __r
 static int dem_window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kDemStageBytes); }
 
 template <int PREC, bool PAIR2, int ADV>
-__global__ void TMD_TILE_KERNEL k_dem_fused(const double4* __restrict__ spos, const double4* __restrict__ svel,
+__global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const double4* __restrict__ spos, const double4* __restrict__ svel,
                                                            const int* __restrict__ start, const int* __restrict__ work,
                                                            const int* __restrict__ wk, const int* __restrict__ tile_start,
                                                            const int* __restrict__ nbc, const nb_t* __restrict__ nbl,

@@ -30,7 +30,7 @@ __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict_
 // Per-atom inputs come in work order (mirror index work_k, cell work_c), so
 // a wavefront's reads of them are contiguous.
 template <bool HALF>
-__global__ void TMD_TILE_KERNEL k_nlist(const double4* __restrict__ spos, const int* __restrict__ work_k,
+__global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __restrict__ spos, const int* __restrict__ work_k,
                                                         const int* __restrict__ work_c,
                                                         const int* __restrict__ start, const int* __restrict__ parts,
                                                         const int* __restrict__ work, const int* __restrict__ tile_start,

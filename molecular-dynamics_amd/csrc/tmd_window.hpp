@@ -25,12 +25,6 @@ namespace tmd {
 constexpr int kTileThreads = 512;  // threads per tile block (default)
 constexpr int kMaxTileThreads = 1024;  // launch bound of the single-box tile kernels
 constexpr int kAutoTileThreads = 768;   // largest automatic block (two blocks per CU at <= 80 VGPRs)
-// Tile kernels: blocks of up to kMaxTileThreads.  Two automatic blocks (up to
-// 12 wavefronts each, 3 per SIMD) share a CU only if a wavefront holds at
-// most 80 VGPRs (6 wavefronts per SIMD, allocated 8 at a time from 512);
-// the FP64 and adjoint bodies would take 82-92 unconstrained and run one
-// block per CU, so the compiler is asked for 6 wavefronts per SIMD.
-#define TMD_TILE_KERNEL __launch_bounds__(kMaxTileThreads) __attribute__((amdgpu_waves_per_eu(6, 8)))
 constexpr int kMaxTile = 8;        // cells per tile dimension, at most
 constexpr int kMaxSeg = (kMaxTile + 2) * (kMaxTile + 2);
 constexpr int kWinMax = 65535;     // window offsets are 16-bit
