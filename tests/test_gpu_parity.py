@@ -722,3 +722,18 @@ def test_dem_step_loop_unstaged_windows(tm):
     assert np.abs(r_st[2]).max() > 1.0  # contacts
     for p, q in zip(r_st, r_un):
         assert np.array_equal(p, q)
+
+
+def test_window_beyond_16bit_offsets_fails_loudly(tm):
+    """List entries are 16-bit offsets into a tile's window: a window of
+    more than 65,535 atoms (here 8-cell tiles of 6.0-wide cells over an FCC
+    liquid-density box, about 180,000 atoms per window) stops the set-up
+    with an error naming the limit instead of building wrong lists."""
+    announce("tile window beyond the 16-bit offset range -> error")
+    m, x, v = tm.generate_fcc(36, 36, 36)
+    sim = tm.Simulation(m, x, v, tm.world_aabb(36, 36, 36), 6.0)
+    sim.set_tile(8)
+    sim.copy_to_accelerator()
+    with pytest.raises(tm.TinyMDError, match="16-bit window range"):
+        sim.setup(False, RC, SKIN)
+    sim.close()
