@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_adjoint(const double4* _
                                                              LJArgs a, AdjCoef c, double4* __restrict__ xbar,
                                                              double* __restrict__ partials) {
     extern __shared__ double dyn[];
-    const int tile = blockIdx.x;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) {
         if (threadIdx.x == 0) {

@@ -146,7 +146,7 @@ template <int SHIFT>
 __global__ void __launch_bounds__(kMaxTileThreads) k_lj_energy(TMD_KERNEL_LIST_PARAMS, LJArgs a, int half,
                                                            double* __restrict__ partials) {
     extern __shared__ double dyn[];
-    const int tile = blockIdx.x;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) {
         if (threadIdx.x == 0) {
@@ -171,9 +171,9 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_energy(TMD_KERNEL_LIST_P
     const double bvir = block_sum_double_any(vir);
     const double bnp = block_sum_double_any(np);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x + 0] = bpe;
-        partials[3 * blockIdx.x + 1] = bvir;
-        partials[3 * blockIdx.x + 2] = bnp;
+        partials[3 * tile + 0] = bpe;  // by tile: the fixed reduction order does not depend on the block mapping
+        partials[3 * tile + 1] = bvir;
+        partials[3 * tile + 2] = bnp;
     }
 }
 
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const double4* __
                                                            double4* __restrict__ svel_next, double4* __restrict__ force,
                                                            double4* __restrict__ vel, double4* __restrict__ pos) {
     extern __shared__ double dyn[];
-    const int tile = blockIdx.x;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
     if (nloc == 0) return;
     __shared__ TileWin W;
