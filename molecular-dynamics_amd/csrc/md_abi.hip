@@ -1120,7 +1120,14 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
         for (int j = 0; j < steps && !has_error(); ++j) {
             hipEvent_t ev = nullptr;
             if ((j + 1) % reneigh_every == 0) {
-                dem_unmirror(s);
+                // the rebuild's ghost selection pass copies the locals'
+                // positions and velocities back from the mirrors (as in the
+                // LJ loop); its ghosts are made anew from them
+                if (s.dem_mirrored) {
+                    s.pos_stale = 1;
+                    s.vel_mirrored = 1;
+                    s.dem_mirrored = 0;
+                }
                 rebuild(s, hf, cutoff + skin);
             } else if (s.dem_mirrored) {
                 s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
