@@ -429,15 +429,16 @@ def hbm_copy_gbs(nbytes=1 << 30, reps=5):
 
 
 def load_profile_figures():
-    """Per-launch figures of the force kernel from the committed rocprofv3
-    summaries: HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated), the
-    FP64 and FP32 VALU flops of its counted instructions, and its average
-    duration in the kernel trace."""
+    """Per-launch figures from the committed rocprofv3 summaries: the LJ
+    force kernel's HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated), the
+    FP64 and FP32 VALU flops of its counted instructions and its average
+    duration in the kernel trace, and the DEM step pass's HBM bytes."""
     import re
     out = {}
     for name, keys in (("force_lj_fp32mixed_80cube_traffic.md", ("traffic_bytes_per_launch",)),
                        ("force_lj_80cube_valu_flops.md", ("valu_f64_flops_per_launch", "valu_f32_flops_per_launch")),
-                       ("lj_80cube_fp32mixed_full_kernel_stats.md", ("trace_avg_us",))):
+                       ("lj_80cube_fp32mixed_full_kernel_stats.md", ("trace_avg_us",)),
+                       ("dem_528k_step_pass_counters.md", ("dem_traffic_bytes_per_launch",))):
         p = os.path.join(ROOT, "profiles", name)
         if not os.path.exists(p):
             continue
@@ -589,6 +590,8 @@ def main():
     if batched:
         line["batched"] = batched
     if dem:
+        # measured HBM bytes of the DEM step pass (copy-calibrated counters)
+        dem["roofline"]["traffic"] = prof.get("dem_traffic_bytes_per_launch")
         line["dem"] = dem
     if shard:
         line["shard_40cube_of_80cube_2x2x2"] = shard
