@@ -36,7 +36,7 @@ struct AdjCoef {
 };
 
 template <int PREC, int SHIFT>
-__global__ void __launch_bounds__(kMaxTileThreads) k_lj_adjoint(const double4* __restrict__ pos, const double4* __restrict__ spos,
+__global__ void __launch_bounds__(kMaxTileThreads) k_lj_adjoint(const double4* __restrict__ pos, const CMir spos,
                                                              const int* __restrict__ start, const int* __restrict__ parts,
                                                              const int* __restrict__ work, const int* __restrict__ tile_start,
                                                              const int* __restrict__ nbc, const nb_t* __restrict__ nbl,
@@ -246,7 +246,7 @@ static void launch_lj_adjoint(Sim& s, double rc, double eps, double sigma, const
     double* part = partials(s, 2 * (size_t)ntiles);
 #define TMD_ADJ(P, SH)                                                                                                  \
     hipLaunchKernelGGL((k_lj_adjoint<P, SH>), dim3(ntiles), dim3(s.tile_threads), lds, s.stream,                          \
-                       (const double4*)s.pos.as<double4>(), (const double4*)s.spos.as<double4>(),                       \
+                       (const double4*)s.pos.as<double4>(), CMir{s.spos.as<double>()},                       \
                        (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), \
                        (const int*)s.tile_start.as<int>(), (const int*)s.nb_count.as<int>(),                            \
                        (const nb_t*)s.nb_list.as<nb_t>(), (const int*)s.g_owner.as<int>(),                             \

@@ -36,7 +36,7 @@ namespace tmd {
 
 
 #define TMD_KERNEL_LIST_PARAMS                                                                                   \
-    const double4 *__restrict__ pos, const double4 *__restrict__ spos, const int *__restrict__ start,              \
+    const double4 *__restrict__ pos, const CMir spos, const int *__restrict__ start,                                \
         const int *__restrict__ parts, const int *__restrict__ work, const int *__restrict__ tile_start,           \
         const int *__restrict__ nbc, const nb_t *__restrict__ nbl
 
@@ -58,8 +58,8 @@ namespace tmd {
 // exactly), so only the summation order differs from the reference's.
 template <int PREC, int SHIFT, int UNIT, int KICK, bool PAIR2>
 __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PARAMS, LJArgs a, double4* __restrict__ force,
-                                                         double4* __restrict__ vel, double hdt, double dt,
-                                                         const int* __restrict__ wk, double4* __restrict__ spos_next,
+                                                         double4* __restrict__ vel, const Mir svel, double hdt, double dt,
+                                                         const int* __restrict__ wk, const Mir spos_next,
                                                          int vel_by_k, const int* __restrict__ nbc2,
                                                          const nb_t* __restrict__ nbl2) {
     extern __shared__ double dyn[];
@@ -86,8 +86,8 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
         if (KICK != 2) force[i] = f;
         if (KICK) {
             // velocity-Verlet final kick (k_final_integration's arithmetic)
-            const int iv = vel_by_k ? kw : i;  // velocities in atom or in mirror order
-            double4 v = vel[iv];
+            // velocities in atom order (vel) or in mirror order (svel)
+            double4 v = vel_by_k ? (double4)svel[kw] : vel[i];
             v.x = v.x + f.x * hdt;
             v.y = v.y + f.y * hdt;
             v.z = v.z + f.z * hdt;
@@ -102,7 +102,8 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
                 x.z = x.z + v.z * dt;
                 spos_next[kw] = x;
             }
-            vel[iv] = v;
+            if (vel_by_k) svel[kw] = v;
+            else vel[i] = v;
         }
     }
 }
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_lj_full(TMD_KERNEL_LIST_PAR
 // (k_final_integration and k_initial_integration arithmetic, in that order).
 template <int ADV>
 __global__ void k_dem_finish(int n, const double4* __restrict__ force, double4* __restrict__ vel, double hdt, double dt,
-                             double4* __restrict__ pos, const int* __restrict__ inv, double4* __restrict__ spos) {
+                             double4* __restrict__ pos, const int* __restrict__ inv, const Mir spos) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 f = force[i];
@@ -343,13 +344,13 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem(TMD_KERNEL_LIST_PARAMS,
 static int dem_window_cap(const Sim& s) { return std::min(s.max_window + 1, kLdsBudget / kDemStageBytes); }
 
 template <int PREC, bool PAIR2, int ADV>
-__global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const double4* __restrict__ spos, const double4* __restrict__ svel,
+__global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const CMir spos, const CMir svel,
                                                            const int* __restrict__ start, const int* __restrict__ work,
                                                            const int* __restrict__ wk, const int* __restrict__ tile_start,
                                                            const int* __restrict__ nbc, const nb_t* __restrict__ nbl,
                                                            const int* __restrict__ nbc2, const nb_t* __restrict__ nbl2,
-                                                           DEMArgs a, double hdt, double dt, double4* __restrict__ spos_next,
-                                                           double4* __restrict__ svel_next, double4* __restrict__ force,
+                                                           DEMArgs a, double hdt, double dt, const Mir spos_next,
+                                                           const Mir svel_next, double4* __restrict__ force,
                                                            double4* __restrict__ vel, double4* __restrict__ pos) {
     extern __shared__ double dyn[];
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -471,11 +472,11 @@ void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double
     s.timer.begin(K_FORCE, s.stream, &ev);
 #define TMD_DEMF(P, H, A)                                                                                               \
     hipLaunchKernelGGL((k_dem_fused<P, H, A>), dim3(nb), dim3(s.tile_threads), lds, s.stream,                            \
-                       (const double4*)s.spos.as<double4>(), (const double4*)s.svel.as<double4>(),                     \
+                       CMir{s.spos.as<double>()}, CMir{s.svel.as<double>()},                                           \
                        (const int*)s.cell_start.as<int>(), (const int*)s.work.as<int>(), (const int*)s.work_k.as<int>(), \
                        (const int*)s.tile_start.as<int>(), (const int*)s.nb_count.as<int>(), (const nb_t*)s.nb_list.as<nb_t>(), \
                        (const int*)s.nb_count2.as<int>(), (const nb_t*)s.nb_list2.as<nb_t>(), a, hdt, dt,               \
-                       s.spos2.as<double4>(), s.svel2.as<double4>(), s.force.as<double4>(), s.vel.as<double4>(),          \
+                       Mir{s.spos2.as<double>()}, Mir{s.svel2.as<double>()}, s.force.as<double4>(), s.vel.as<double4>(),    \
                        s.pos.as<double4>())
 #define TMD_DEMF_A(P, H) do { if (advance) TMD_DEMF(P, H, 1); else TMD_DEMF(P, H, 0); } while (0)
 #define TMD_DEMF_H(P) do { if (half) TMD_DEMF_A(P, true); else TMD_DEMF_A(P, false); } while (0)
@@ -497,7 +498,7 @@ void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double
 }
 
 #define TMD_LIST_ARGS                                                                                              \
-    (const double4*)s.pos.as<double4>(), (const double4*)s.spos.as<double4>(), (const int*)s.cell_start.as<int>(), \
+    (const double4*)s.pos.as<double4>(), CMir{s.spos.as<double>()}, (const int*)s.cell_start.as<int>(),           \
         (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), (const int*)s.tile_start.as<int>(),     \
         (const int*)s.nb_count.as<int>(), (const nb_t*)s.nb_list.as<nb_t>()
 
@@ -524,14 +525,15 @@ void compute_lj(Sim& s, int half, double rc, double eps, double sigma, double ki
         const int unit = (eps == 1.0 && a.sigma6 == 1.0) ? 1 : 0;
         const double hdt = 0.5 * kick_dt;
         // inside the fused step loop the velocities live in mirror order
-        double4* V = s.vel_mirrored ? s.svel.as<double4>() : s.vel.as<double4>();
+        double4* V = s.vel.as<double4>();
+        const Mir SV{s.svel.as<double>()};  // used when the velocities live in the mirror (VK)
         const int VK = s.vel_mirrored;
         const int* WK = s.work_k.as<int>();
-        double4* SN = s.spos2.as<double4>();
+        const Mir SN{s.spos2.as<double>()};
         const int* C2 = s.nb_count2.as<int>();
         const nb_t* L2 = s.nb_list2.as<nb_t>();
 #define TMD_LJ_K(P, SH, U, K, H)                                                                                       \
-    hipLaunchKernelGGL((k_lj_full<P, SH, U, K, H>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, hdt, \
+    hipLaunchKernelGGL((k_lj_full<P, SH, U, K, H>), dim3(nb), dim3(s.tile_threads), lds, s.stream, TMD_LIST_ARGS, a, F, V, SV, hdt, \
                        kick_dt, WK, SN, VK, C2, L2)
 #define TMD_LJ_H(P, SH, U, K)                                                                                          \
     do { if (half) TMD_LJ_K(P, SH, U, K, true); else TMD_LJ_K(P, SH, U, K, false); } while (0)
@@ -598,7 +600,7 @@ void compute_dem(Sim& s, int half, double rc, double gn, double gt, double k, do
         double4* V2 = s.vel.as<double4>();
         double4* X = s.pos.as<double4>();
         const int* I = s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr;
-        double4* M = s.spos.as<double4>();
+        const Mir M{s.spos.as<double>()};
         const double hdt = 0.5 * kick_dt;
         const dim3 g(div_up(s.n, 256)), b(256);
         if (advance) hipLaunchKernelGGL(k_dem_finish<1>, g, b, 0, s.stream, s.n, (const double4*)F, V2, hdt, kick_dt, X, I, M);

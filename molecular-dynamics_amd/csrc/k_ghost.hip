@@ -52,20 +52,20 @@ struct FaceArgs {
 // through the mirror index inv).
 __global__ void __launch_bounds__(kScanThreads) k_face_flags(double4* __restrict__ pos, FaceArgs a, unsigned char* __restrict__ face,
                                                              int* __restrict__ totals, const int* __restrict__ inv,
-                                                             const double4* __restrict__ mpos,
-                                                             const double4* __restrict__ mvel, double4* __restrict__ vel) {
+                                                             const CMir mpos, const CMir mvel,
+                                                             double4* __restrict__ vel) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int t[6] = {0, 0, 0, 0, 0, 0};
     if (i < a.n) {
         double4 p;
-        if (mpos || mvel) {
+        if (mpos.p || mvel.p) {
             const int k = inv[i];
-            if (mpos) p = mpos[k];
-            if (mvel) vel[i] = mvel[k];
+            if (mpos.p) p = mpos[k];
+            if (mvel.p) vel[i] = mvel[k];
         }
-        if (!mpos) p = pos[i];
+        if (!mpos.p) p = pos[i];
         if (a.wrap) pbc_wrap(p, a.pbc);  // md_enforce_pbc, fused
-        if (a.wrap || mpos) pos[i] = p;
+        if (a.wrap || mpos.p) pos[i] = p;
         const double c[3] = {p.x, p.y, p.z};
         int bits = 0;
 #pragma unroll
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(kScanThreads) k_border_scatter(double4* __rest
 // chained construction because each coordinate receives exactly one shift).
 __global__ void k_ghost_refresh(double4* __restrict__ pos, double4* __restrict__ vel, const int* __restrict__ g_owner,
                                 const int* __restrict__ g_code, int n, int ng, double L0, double L1, double L2, int with_vel,
-                                const int* __restrict__ inv, double4* __restrict__ spos) {
+                                const int* __restrict__ inv, const Mir spos) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int o = g_owner[g];
@@ -220,7 +220,7 @@ __global__ void k_ghost_refresh(double4* __restrict__ pos, double4* __restrict__
 // entry (inv >= 0) is rewritten.
 __global__ void k_step_refresh(double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
                                int n, int ng, double L0, double L1, double L2, const int* __restrict__ inv,
-                               double4* __restrict__ spos) {
+                               const Mir spos) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n + ng) return;
     double4 q;
@@ -246,7 +246,7 @@ __global__ void k_step_refresh(double4* __restrict__ pos, const int* __restrict_
 // owner read from its mirror entry; written to pos and to the ghost's
 // mirror entry (owners are locals, so no entry is both read and written).
 __global__ void k_ghost_step(double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code, int n,
-                             int ng, double L0, double L1, double L2, const int* __restrict__ inv, double4* __restrict__ spos) {
+                             int ng, double L0, double L1, double L2, const int* __restrict__ inv, const Mir spos) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     int ix, iy, iz;
@@ -290,7 +290,7 @@ void build_ghost_src(Sim& s) {
 // sync_pos_from_mirror.
 __global__ void k_ghost_step_mirror(const int* __restrict__ g_ko, const int* __restrict__ g_kg,
                                     const int* __restrict__ g_code, int ng, double L0, double L1, double L2,
-                                    double4* __restrict__ spos) {
+                                    const Mir spos) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int kg = g_kg[g];
@@ -310,7 +310,7 @@ __global__ void k_ghost_step_mirror(const int* __restrict__ g_ko, const int* __r
 // image shift, velocity copied), k_ghost_refresh's arithmetic.
 __global__ void k_ghost_step_mirror_v(const int* __restrict__ g_ko, const int* __restrict__ g_kg,
                                       const int* __restrict__ g_code, int ng, double L0, double L1, double L2,
-                                      double4* __restrict__ spos, double4* __restrict__ svel) {
+                                      const Mir spos, const Mir svel) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int kg = g_kg[g];
@@ -334,7 +334,7 @@ void dem_ghost_step(Sim& s) {
     const int nt = 256;
     hipLaunchKernelGGL(k_ghost_step_mirror_v, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, (const int*)s.g_ko.as<int>(),
                        (const int*)s.g_kg.as<int>(), (const int*)s.g_code.as<int>(), s.ng, s.g.len[0], s.g.len[1], s.g.len[2],
-                       s.spos.as<double4>(), s.svel.as<double4>());
+                       Mir{s.spos.as<double>()}, Mir{s.svel.as<double>()});
     TMD_HIP(hipGetLastError());
 }
 
@@ -357,7 +357,7 @@ __global__ void k_ghost_pack_lag(const double4* __restrict__ pos, const int* __r
 
 __global__ void k_ghost_unpack_lag(double4* __restrict__ pos, double4* __restrict__ vel, const int* __restrict__ g_owner,
                                    const double4* __restrict__ buf, int n, int ng, int with_vel, const int* __restrict__ inv,
-                                   double4* __restrict__ spos) {
+                                   const Mir spos) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     pos[n + g] = buf[g];
@@ -380,8 +380,8 @@ void build_ghosts(Sim& s, bool wrap) {
     // after the fused step loop the locals' positions and velocities live in
     // the mirror: the selection pass copies them back (binning's inv is still
     // that of the mirror)
-    const double4* mpos = s.pos_stale ? s.spos.as<double4>() : nullptr;
-    const double4* mvel = s.vel_mirrored ? s.svel.as<double4>() : nullptr;
+    const CMir mpos{s.pos_stale ? s.spos.as<double>() : nullptr};
+    const CMir mvel{s.vel_mirrored ? s.svel.as<double>() : nullptr};
     s.pos_stale = 0;
     s.vel_mirrored = 0;
     s.ghost_pos_stale = 0;  // the ghosts are made anew
@@ -460,26 +460,26 @@ void refresh_ghosts(Sim& s) {
                            s.gbuf.as<double4>());
         hipLaunchKernelGGL(k_ghost_unpack_lag, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.vel.as<double4>(),
                            (const int*)s.g_owner.as<int>(), (const double4*)s.gbuf.as<double4>(), s.n, s.ng, with_vel,
-                           s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr, s.spos.as<double4>());
+                           s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr, Mir{s.spos.as<double>()});
     } else {
         hipLaunchKernelGGL(k_ghost_refresh, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(), s.vel.as<double4>(),
                            (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0], s.g.len[1],
                            s.g.len[2], with_vel, s.inv_valid ? (const int*)s.cell_inv.as<int>() : nullptr,
-                           s.spos.as<double4>());
+                           Mir{s.spos.as<double>()});
     }
     TMD_HIP(hipGetLastError());
 }
 
 // pos[parts[k]] = mirror[k] for the locals of the cell CSR (when pos is
 // given), and likewise the velocities from their mirror order (when svel is)
-__global__ void k_unmirror(const double4* __restrict__ spos, const int* __restrict__ parts, int nbinned, int n,
-                           double4* __restrict__ pos, const double4* __restrict__ svel, double4* __restrict__ vel) {
+__global__ void k_unmirror(const CMir spos, const int* __restrict__ parts, int nbinned, int n,
+                           double4* __restrict__ pos, const CMir svel, double4* __restrict__ vel) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nbinned) return;
     const int j = parts[k];
     if (j < n) {
         if (pos) pos[j] = spos[k];
-        if (svel) vel[j] = svel[k];
+        if (svel.p) vel[j] = svel[k];
     }
 }
 
@@ -491,17 +491,17 @@ void sync_pos_from_mirror(Sim& s) {
         if (s.ng > 0)
             hipLaunchKernelGGL(k_ghost_step, dim3(div_up(s.ng, 256)), dim3(256), 0, s.stream, s.pos.as<double4>(),
                                (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0],
-                               s.g.len[1], s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+                               s.g.len[1], s.g.len[2], (const int*)s.cell_inv.as<int>(), Mir{s.spos.as<double>()});
         TMD_HIP(hipGetLastError());
         s.ghost_pos_stale = 0;
     }
     if (!s.pos_stale && !s.vel_mirrored) return;
     const int nt = 256;
     if (s.nbinned > 0)
-        hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.spos.as<double4>(),
+        hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, CMir{s.spos.as<double>()},
                            (const int*)s.cell_parts.as<int>(), s.nbinned, s.n,
                            s.pos_stale ? s.pos.as<double4>() : (double4*)nullptr,
-                           s.vel_mirrored ? (const double4*)s.svel.as<double4>() : (const double4*)nullptr, s.vel.as<double4>());
+                           CMir{s.vel_mirrored ? s.svel.as<double>() : nullptr}, s.vel.as<double4>());
     TMD_HIP(hipGetLastError());
     s.pos_stale = 0;
     s.vel_mirrored = 0;
@@ -509,7 +509,7 @@ void sync_pos_from_mirror(Sim& s) {
 
 // velocities of the locals between atom order (vel) and mirror order (svel)
 __global__ void k_vel_mirror(const int* __restrict__ work, const int* __restrict__ wk, int n, double4* __restrict__ vel,
-                             double4* __restrict__ svel, int into_mirror) {
+                             const Mir svel, int into_mirror) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
     if (into_mirror) svel[wk[w]] = vel[work[w]];
@@ -523,12 +523,12 @@ void mirror_velocities(Sim& s, bool into_mirror) {
     if (has_error() || s.n == 0) return;
     if (into_mirror == (s.vel_mirrored != 0)) return;
     if (into_mirror) {
-        s.svel.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+        s.svel.reserve((size_t)kMirBytes * (size_t)std::max(s.nbinned, 1), false, s.stream);
         if (has_error()) return;
     }
     const int nt = 256;
     hipLaunchKernelGGL(k_vel_mirror, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, (const int*)s.work.as<int>(),
-                       (const int*)s.work_k.as<int>(), s.n, s.vel.as<double4>(), s.svel.as<double4>(), into_mirror ? 1 : 0);
+                       (const int*)s.work_k.as<int>(), s.n, s.vel.as<double4>(), Mir{s.svel.as<double>()}, into_mirror ? 1 : 0);
     TMD_HIP(hipGetLastError());
     s.vel_mirrored = into_mirror ? 1 : 0;
 }
@@ -546,18 +546,18 @@ void step_refresh(Sim& s) {
         if (s.ng > 0)
             hipLaunchKernelGGL(k_ghost_step_mirror, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream,
                                (const int*)s.g_ko.as<int>(), (const int*)s.g_kg.as<int>(), (const int*)s.g_code.as<int>(),
-                               s.ng, s.g.len[0], s.g.len[1], s.g.len[2], s.spos.as<double4>());
+                               s.ng, s.g.len[0], s.g.len[1], s.g.len[2], Mir{s.spos.as<double>()});
         s.ghost_pos_stale = 1;
     } else if (s.ghost_stale) {
         // locals are current in the mirror: rewrite the ghosts only
         if (s.ng > 0)
             hipLaunchKernelGGL(k_ghost_step, dim3(div_up(s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
                                (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0],
-                               s.g.len[1], s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+                               s.g.len[1], s.g.len[2], (const int*)s.cell_inv.as<int>(), Mir{s.spos.as<double>()});
     } else {
         hipLaunchKernelGGL(k_step_refresh, dim3(div_up(s.n + s.ng, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
                            (const int*)s.g_owner.as<int>(), (const int*)s.g_code.as<int>(), s.n, s.ng, s.g.len[0], s.g.len[1],
-                           s.g.len[2], (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+                           s.g.len[2], (const int*)s.cell_inv.as<int>(), Mir{s.spos.as<double>()});
     }
     TMD_HIP(hipGetLastError());
     s.ghost_stale = 0;
@@ -592,9 +592,9 @@ void dem_unmirror(Sim& s) {
     s.dem_mirrored = 0;
     const int nt = 256;
     if (s.nbinned > 0)
-        hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, (const double4*)s.spos.as<double4>(),
+        hipLaunchKernelGGL(k_unmirror, dim3(div_up(s.nbinned, nt)), dim3(nt), 0, s.stream, CMir{s.spos.as<double>()},
                            (const int*)s.cell_parts.as<int>(), s.nbinned, s.n, s.pos.as<double4>(),
-                           (const double4*)s.svel.as<double4>(), s.vel.as<double4>());
+                           CMir{s.svel.as<double>()}, s.vel.as<double4>());
     TMD_HIP(hipGetLastError());
     refresh_ghosts(s);  // also writes the ghosts' mirror entries, so the mirror stays that of pos
 }

@@ -10,7 +10,7 @@ namespace tmd {
 // mirror is kept current too (spos[inv[i]] = x)
 template <int MIRROR>
 __global__ void k_initial_integration(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ force,
-                                      int n, double hdt, double dt, const int* __restrict__ inv, double4* __restrict__ spos) {
+                                      int n, double hdt, double dt, const int* __restrict__ inv, const Mir spos) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 v = vel[i];
@@ -69,11 +69,11 @@ void initial_integration(Sim& s, double dt) {
     if (s.inv_valid)
         hipLaunchKernelGGL(k_initial_integration<1>, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
                            s.vel.as<double4>(), (const double4*)s.force.as<double4>(), s.n, 0.5 * dt, dt,
-                           (const int*)s.cell_inv.as<int>(), s.spos.as<double4>());
+                           (const int*)s.cell_inv.as<int>(), Mir{s.spos.as<double>()});
     else
         hipLaunchKernelGGL(k_initial_integration<0>, dim3(div_up(s.n, nt)), dim3(nt), 0, s.stream, s.pos.as<double4>(),
                            s.vel.as<double4>(), (const double4*)s.force.as<double4>(), s.n, 0.5 * dt, dt, (const int*)nullptr,
-                           (double4*)nullptr);
+                           Mir{nullptr});
     if (!s.inv_valid) mirror_invalidate(s);
     s.timer.end(K_INTEGRATE_1, s.stream, ev);
     TMD_HIP(hipGetLastError());

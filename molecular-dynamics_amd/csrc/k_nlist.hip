@@ -18,19 +18,19 @@ namespace tmd {
 // Cell-ordered position mirror: spos[k] = pos[parts[k]] (locals and ghosts);
 // with svel, also the locals' velocities in mirror order.
 __global__ void k_mirror(const double4* __restrict__ pos, const int* __restrict__ parts, const int* __restrict__ nbinned,
-                         double4* __restrict__ spos, int n, const double4* __restrict__ vel, double4* __restrict__ svel,
+                         const Mir spos, int n, const double4* __restrict__ vel, const Mir svel,
                          int vel_ghosts) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *nbinned) return;
     const int j = parts[k];
     spos[k] = pos[j];
-    if (svel && (j < n || vel_ghosts)) svel[k] = vel[j];
+    if (svel.p && (j < n || vel_ghosts)) svel[k] = vel[j];
 }
 
 // Per-atom inputs come in work order (mirror index work_k, cell work_c), so
 // a wavefront's reads of them are contiguous.
 template <bool HALF>
-__global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const double4* __restrict__ spos, const int* __restrict__ work_k,
+__global__ void __launch_bounds__(kMaxTileThreads) k_nlist(const CMir spos, const int* __restrict__ work_k,
                                                         const int* __restrict__ work_c,
                                                         const int* __restrict__ start, const int* __restrict__ parts,
                                                         const int* __restrict__ work, const int* __restrict__ tile_start,
@@ -95,8 +95,8 @@ static void apply_window_stats(Sim& s, const int* host) {
     s.max_window = host[2];
     s.tile_threads = tile_block_threads(s.tile_threads_req, host[3]);
     retune_tile_edge(s);
-    s.spos.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
-    s.spos2.reserve(sizeof(double4) * (size_t)std::max(s.nbinned, 1), false, s.stream);
+    s.spos.reserve((size_t)kMirBytes * (size_t)std::max(s.nbinned, 1), false, s.stream);
+    s.spos2.reserve((size_t)kMirBytes * (size_t)std::max(s.nbinned, 1), false, s.stream);
     if (has_error()) return;
     if (s.max_window > kWinMax) {
         char msg[200];
@@ -137,8 +137,8 @@ void build_windows(Sim& s) {
     TMD_HIP(hipMemcpyAsync(&host[2], counters + 6, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
     if (defer) {
         const int bound = std::max(s.n + s.ng, 1);  // binned atoms, at most
-        s.spos.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
-        s.spos2.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+        s.spos.reserve((size_t)kMirBytes * (size_t)bound, false, s.stream);
+        s.spos2.reserve((size_t)kMirBytes * (size_t)bound, false, s.stream);
         if (has_error()) return;
         s.windows_pending = 1;
         s.windows_valid = 1;
@@ -162,14 +162,14 @@ void finish_windows(Sim& s) {
 void dem_mirror(Sim& s) {
     const int bound = s.n + s.ng;
     if (has_error() || bound == 0) return;
-    s.svel.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
-    s.svel2.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
-    s.spos2.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+    s.svel.reserve((size_t)kMirBytes * (size_t)bound, false, s.stream);
+    s.svel2.reserve((size_t)kMirBytes * (size_t)bound, false, s.stream);
+    s.spos2.reserve((size_t)kMirBytes * (size_t)bound, false, s.stream);
     if (has_error()) return;
     const int nt = 256;
     hipLaunchKernelGGL(k_mirror, dim3(div_up(bound, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
                        (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
-                       s.spos.as<double4>(), s.n, (const double4*)s.vel.as<double4>(), s.svel.as<double4>(), 1);
+                       Mir{s.spos.as<double>()}, s.n, (const double4*)s.vel.as<double4>(), Mir{s.svel.as<double>()}, 1);
     TMD_HIP(hipGetLastError());
     s.mirror_fresh = 1;
     s.dem_mirrored = 1;
@@ -184,13 +184,13 @@ void refresh_mirror(Sim& s) {
     // inside the fused step loop the velocities follow the new order too
     const bool with_vel = s.vel_mirror_wanted && !s.vel_mirrored && s.n > 0;
     if (with_vel) {
-        s.svel.reserve(sizeof(double4) * (size_t)bound, false, s.stream);
+        s.svel.reserve((size_t)kMirBytes * (size_t)bound, false, s.stream);
         if (has_error()) return;
     }
     hipLaunchKernelGGL(k_mirror, dim3(div_up(bound, nt)), dim3(nt), 0, s.stream, (const double4*)s.pos.as<double4>(),
                        (const int*)s.cell_parts.as<int>(), (const int*)(s.cell_start.as<int>() + s.g.ncells),
-                       s.spos.as<double4>(), s.n, (const double4*)s.vel.as<double4>(),
-                       with_vel ? s.svel.as<double4>() : (double4*)nullptr, 0);
+                       Mir{s.spos.as<double>()}, s.n, (const double4*)s.vel.as<double4>(),
+                       Mir{with_vel ? s.svel.as<double>() : nullptr}, 0);
     TMD_HIP(hipGetLastError());
     s.mirror_fresh = 1;
     if (with_vel) s.vel_mirrored = 1;
@@ -226,7 +226,7 @@ void build_neighbor_lists(Sim& s, int half, double cutoff) {
         const size_t lds = (size_t)wcap * sb + rsb;
         if (s.n > 0) {
 #define TMD_NLIST(H)                                                                                                   \
-    hipLaunchKernelGGL(k_nlist<H>, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, (const double4*)s.spos.as<double4>(), \
+    hipLaunchKernelGGL(k_nlist<H>, dim3(ntiles), dim3(s.tile_threads), lds, s.stream, CMir{s.spos.as<double>()},           \
                        (const int*)s.work_k.as<int>(), (const int*)s.work_c.as<int>(),                                      \
                        (const int*)s.cell_start.as<int>(), (const int*)s.cell_parts.as<int>(), (const int*)s.work.as<int>(), \
                        (const int*)s.tile_start.as<int>(), a, s.nb_count.as<int>(), s.nb_list.as<nb_t>(), \

@@ -140,7 +140,7 @@ struct Sim {
     DevBuf cell_inv;                   // CSR index of each atom (-1: outside the grid)
     DevBuf work_start, work;           // per-tile local counts; locals in tile work order (work slots)
     DevBuf tile_start;                 // first work slot of each tile [ntiles + 1]
-    DevBuf spos;                       // cell-ordered position mirror (double4)
+    DevBuf spos;                       // cell-ordered position mirror (x, y, z packed, 24 B per atom)
     DevBuf spos2;                      // second mirror: the force pass that advances positions writes the
                                        // new local positions here, then the two are swapped
     DevBuf work_k;                     // mirror (CSR) index of each work slot's atom

@@ -70,8 +70,8 @@ __host__ __device__ __forceinline__ double kahan_value(float s, float c) { retur
 
 // Neighbour position of window offset t: LDS (x, y, z interleaved) when the
 // window is staged, else the cell-ordered mirror.
-template <bool STAGED>
-__device__ __forceinline__ void win_pos(int t, const WinStage& st, const double4* __restrict__ spos, const TileWin& W,
+template <bool STAGED, class SP>
+__device__ __forceinline__ void win_pos(int t, const WinStage& st, const SP& spos, const TileWin& W,
                                         double& x, double& y, double& z) {
     if (STAGED) {
         x = st.xyz[3 * t + 0];
@@ -100,9 +100,9 @@ __device__ __forceinline__ void win_pos(int t, const WinStage& st, const double4
 // followed by column 2's (cnt2 at lp2), one running sum over both; a
 // wavefront then runs as many groups as its busiest lane has in both
 // columns together, not the busiest lane of each column in turn.
-template <int PREC, int SHIFT, int UNIT, bool STAGED, bool PAIR2 = false>
+template <int PREC, int SHIFT, int UNIT, bool STAGED, bool PAIR2 = false, class SP = const double4*>
 __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const WinStage& sp,
-                                                const double4* __restrict__ spos, const TileWin& W, const LJArgs& a,
+                                                const SP& spos, const TileWin& W, const LJArgs& a,
                                                 int cnt2 = 0, const int4* __restrict__ lp2 = nullptr) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     float sxa = 0.f, sya = 0.f, sza = 0.f, cxa = 0.f, cya = 0.f, cza = 0.f;
@@ -196,9 +196,9 @@ __device__ __forceinline__ double4 lj_full_atom(const double4 pi, int cnt, const
 // 1/2; half lists weight local-local entries 1 and local-ghost entries 1/2
 // (an index j >= nloc_end is a ghost).  Entries are read 8 at a time (one
 // 16-byte load per group, as in the force pass) and summed in list order.
-template <int SHIFT, bool STAGED>
+template <int SHIFT, bool STAGED, class SP>
 __device__ __forceinline__ void lj_energy_atom(const double4 pi, int cnt, const int4* __restrict__ lp, const WinStage& sp,
-                                               const double4* __restrict__ spos, const int* __restrict__ parts,
+                                               const SP& spos, const int* __restrict__ parts,
                                                const TileWin& W, const LJArgs& a, int half, int nloc_end, double& pe,
                                                double& vir, double& np) {
     const int ng = (cnt + 7) >> 3;
@@ -306,9 +306,9 @@ struct ListCol {
 // reactions the half list would have sent to i.  STAGED is a template
 // parameter so that the staged loop addresses LDS directly (ds_read) rather
 // than through generic pointers.
-template <bool HALF, bool STAGED>
+template <bool HALF, bool STAGED, class SP>
 __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const double4 pi, const int* cst, const NbArgs& a,
-                                          const TileWin& W, const WinStage& sp, const double4* __restrict__ spos,
+                                          const TileWin& W, const WinStage& sp, const SP& spos,
                                           const int* __restrict__ parts, nb_t* __restrict__ nb_list,
                                           nb_t* __restrict__ nb_list2, int& cnt2) {
     const int cx = c % a.g.nx, cy = (c / a.g.nx) % a.g.ny, cz = c / (a.g.nx * a.g.ny);

@@ -38,6 +38,33 @@ constexpr int kDemStageBytes = 48;    // DEM step pass: position and velocity, 2
 // LDS) less the static tile table
 constexpr int kDemLdsShare = 160 * 1024 / 3 - 1024;
 
+// The cell-ordered mirrors of the single-box path (positions, velocities)
+// hold x, y, z doubles packed at 24 B per atom, a quarter less than double4
+// records.  These views read and write them as double4 values (w = 0).
+struct MirRef {
+    double* q;
+    __device__ __forceinline__ operator double4() const { return make_double4(q[0], q[1], q[2], 0.0); }
+    __device__ __forceinline__ void operator=(const double4& v) const {
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
+    }
+    // element copy (not a rebinding of the view)
+    __device__ __forceinline__ void operator=(const MirRef& o) const { *this = (double4)o; }
+};
+struct Mir {
+    double* p;
+    __device__ __forceinline__ MirRef operator[](size_t k) const { return MirRef{p + 3 * k}; }
+};
+struct CMir {
+    const double* p;
+    __device__ __forceinline__ double4 operator[](size_t k) const {
+        const double* q = p + 3 * k;
+        return make_double4(q[0], q[1], q[2], 0.0);
+    }
+};
+constexpr int kMirBytes = 24;  // bytes per atom of a mirror
+
 typedef unsigned short nb_t;
 
 // u16 list layout: tiles of 64 work slots, entry k of column l at
@@ -141,7 +168,8 @@ __device__ __forceinline__ void stage_row_starts(const TileWin& W, const int* __
 
 // Stage the window's FP64 positions and, when sj != nullptr, atom indices
 // into LDS if it fits wcap atoms; otherwise consumers read the mirror.
-__device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const double4* __restrict__ spos,
+template <class SP>
+__device__ __forceinline__ bool stage_tile(const TileWin& W, int wcap, const SP& spos,
                                            const int* __restrict__ parts, double* sx, double* sy, double* sz, int* sj) {
     const int S = win_size(W);
     const bool staged = S <= wcap;
@@ -197,7 +225,8 @@ constexpr int kListPad = 3;
 // `pad` sentinel slots (offsets S .. S + pad - 1) into LDS if they fit wcap
 // atoms.  Whole block.  Returns whether it was staged; otherwise consumers
 // read the mirror.
-__device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const double4* __restrict__ spos,
+template <class SP>
+__device__ __forceinline__ bool stage_window(const TileWin& W, int wcap, const SP& spos,
                                              const int* __restrict__ parts, const WinStage& st, int pad = 1) {
     const int S = win_size(W);
     const bool staged = S + pad <= wcap;
