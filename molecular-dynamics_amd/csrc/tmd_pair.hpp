@@ -256,27 +256,65 @@ struct NbArgs {
 };
 
 // One list column being written: groups of 8 entries assembled in registers
-// by funnel shifts (after 8 appends the oldest entry sits in the low half of
-// q0) and stored whole, one 16-byte store per group; groups beyond the
-// capacity are counted, not stored.
+// (q0..q3 hold the last 8 entries, the newest in the high half of q3, and p
+// the two before them) and stored whole, one 16-byte store per group; groups
+// beyond the capacity are counted, not stored.
 struct ListCol {
     int4* col;
     int ngroups;
-    unsigned q0, q1, q2, q3;
+    unsigned p, q0, q1, q2, q3;
     int cnt;
-    __device__ __forceinline__ ListCol(int4* c, int ng) : col(c), ngroups(ng), q0(0u), q1(0u), q2(0u), q3(0u), cnt(0) {}
+    __device__ __forceinline__ ListCol(int4* c, int ng)
+        : col(c), ngroups(ng), p(0u), q0(0u), q1(0u), q2(0u), q3(0u), cnt(0) {}
     __device__ __forceinline__ void shift_in(unsigned t) {
         q0 = __builtin_amdgcn_alignbit(q1, q0, 16);
         q1 = __builtin_amdgcn_alignbit(q2, q1, 16);
         q2 = __builtin_amdgcn_alignbit(q3, q2, 16);
         q3 = __builtin_amdgcn_alignbit(t, q3, 16);
     }
+    __device__ __forceinline__ void store(unsigned d0, unsigned d1, unsigned d2, unsigned d3) {
+        const int grp = (cnt >> 3) - 1;
+        if (grp < ngroups) col[(size_t)grp * 64] = make_int4((int)d0, (int)d1, (int)d2, (int)d3);
+    }
+    // one entry (p is not kept up to date here: push2 refills it before any
+    // store reads it)
     __device__ __forceinline__ void push(unsigned t) {
         shift_in(t);
         ++cnt;
-        if ((cnt & 7) == 0) {
-            const int grp = (cnt >> 3) - 1;
-            if (grp < ngroups) col[(size_t)grp * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+        if ((cnt & 7) == 0) store(q0, q1, q2, q3);
+    }
+    // two entries, t1 first: whole dwords move, no funnel shifts.  A group
+    // completes either at t2 (the last 8 entries, q0..q3) or at t1 (the 8
+    // before t2, one entry lower: the high half of p through the low half
+    // of q3).
+    __device__ __forceinline__ void push2(unsigned t1, unsigned t2) {
+        p = q0;
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = t1 | (t2 << 16);
+        cnt += 2;
+        if ((cnt & 6) == 0) {
+            if (cnt & 1)
+                store(__builtin_amdgcn_alignbit(q0, p, 16), __builtin_amdgcn_alignbit(q1, q0, 16),
+                      __builtin_amdgcn_alignbit(q2, q1, 16), __builtin_amdgcn_alignbit(q3, q2, 16));
+            else
+                store(q0, q1, q2, q3);
+        }
+    }
+    // the set bits of m (bit b: entry base + b) in ascending order: an odd
+    // one alone, then two per step
+    __device__ __forceinline__ void push_mask(unsigned m, unsigned base) {
+        if (__builtin_popcount(m) & 1) {
+            push(base + (unsigned)__builtin_ctz(m));
+            m &= m - 1u;
+        }
+        while (m != 0u) {
+            const unsigned b1 = (unsigned)__builtin_ctz(m);
+            m &= m - 1u;
+            const unsigned b2 = (unsigned)__builtin_ctz(m);
+            m &= m - 1u;
+            push2(base + b1, base + b2);
         }
     }
     // the last group's free slots hold the window's sentinel offset
@@ -296,8 +334,8 @@ struct ListCol {
 // accepted); i itself is removed from the mask by its window offset, so full
 // lists read no atom index (half lists read the staged index for the j < i /
 // ghost filter).  The set bits are then appended in ascending order, which is
-// the reference's order (ListCol).  Only accepted candidates pay for the
-// packing.  Returns the count.
+// the reference's order (ListCol, two entries per step).  Only accepted
+// candidates pay for the packing.  Returns the count.
 //
 // HALF also writes the complementary column (nb_list2, count in cnt2): the
 // local neighbours j > i that the half list leaves to j's own list.  The
@@ -325,8 +363,9 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
         const int t0 = cst[s * per + xo], t1 = cst[s * per + xo + 3];
         for (int tc = t0; tc < t1; tc += 32) {
             const int te = min(tc + 32, t1);
-            // each batch of 4 candidates is shifted in at the low end (first
-            // candidate highest), which needs only small constant bits
+            // the candidates' bits are shifted in at the low end (first
+            // candidate highest): half lists a batch of 4 at a time, full
+            // lists one sign bit per candidate (one funnel shift)
             unsigned m = 0u, m2 = 0u;
             const int nbat = (te - tc + 3) >> 2;  // batches (the loop test is then one compare)
             for (int q = 0; q < nbat; ++q) {
@@ -351,16 +390,25 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
                     }
                     const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                     const double rsq = dx * dx + dy * dy + dz * dz;
-                    bool ok = rsq <= a.cut2;
                     if (HALF) {
+                        bool ok = rsq <= a.cut2;
                         bq2 |= (ok && j < a.n && j > i) ? 8u >> u : 0u;
                         ok = ok && ((a.half_mode == HALF_REF_LITERAL) ? (i > j) : (j >= a.n || i > j));
+                        bq |= ok ? 8u >> u : 0u;
+                    } else {
+                        // the sign of cut2 - rsq is set exactly when rsq > cut2
+                        // (the difference of two equal finite doubles is +0);
+                        // it is shifted in at the low end
+                        const double d = a.cut2 - rsq;
+                        m = __builtin_amdgcn_alignbit(m, (unsigned)__double2hiint(d), 31);
                     }
-                    bq |= ok ? 8u >> u : 0u;
                 }
-                m = (m << 4) | bq;
-                if (HALF) m2 = (m2 << 4) | bq2;
+                if (HALF) {
+                    m = (m << 4) | bq;
+                    m2 = (m2 << 4) | bq2;
+                }
             }
+            if (!HALF) m = ~m;  // the bits were set for the rejected candidates
             // reversed, bit b is candidate tc + b (nbat >= 1: the chunk is not empty)
             m = __builtin_bitreverse32(m) >> (32 - 4 * nbat);
             if (HALF) m2 = __builtin_bitreverse32(m2) >> (32 - 4 * nbat);
@@ -368,17 +416,8 @@ __device__ __forceinline__ int nlist_atom(int i, int w, int c, int k_i, const do
             m &= chunk;
             const unsigned d = (unsigned)(t_i - tc);
             m &= ~(d < 32u ? 1u << d : 0u);  // i is not its own neighbour
-            while (m != 0u) {
-                A.push((unsigned)(tc + __builtin_ctz(m)));
-                m &= m - 1u;
-            }
-            if (HALF) {
-                m2 &= chunk;  // j > i excludes i itself
-                while (m2 != 0u) {
-                    B.push((unsigned)(tc + __builtin_ctz(m2)));
-                    m2 &= m2 - 1u;
-                }
-            }
+            A.push_mask(m, (unsigned)tc);
+            if (HALF) B.push_mask(m2 & chunk, (unsigned)tc);  // j > i excludes i itself
         }
     }
     const unsigned pad = (unsigned)win_size(W);
