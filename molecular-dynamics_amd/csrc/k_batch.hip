@@ -76,7 +76,7 @@ __global__ void k_b_fill(int* __restrict__ p, int n, int v) {
 // integration and periodic wrap (k_integrate.hip arithmetic)
 __global__ void k_b_int1(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, double4* __restrict__ pos,
                          double4* __restrict__ vel, const double4* __restrict__ force, double hdt, double dt,
-                         const int* __restrict__ inv, double4* __restrict__ spos) {
+                         const int* __restrict__ inv, const Mir spos) {
     const int r = eb_rep[blockIdx.x];
     const int i = (blockIdx.x - R[r].eb0) * kBlk + threadIdx.x;
     if (i >= R[r].n) return;
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(kScanThreads) k_b_border_scatter(const RepDesc
 // pos and the ghost's mirror entry (k_ghost_step's arithmetic)
 __global__ void k_b_ghost_step(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep, const int* __restrict__ dyn,
                                double4* __restrict__ pos, const int* __restrict__ g_owner, const int* __restrict__ g_code,
-                               const int* __restrict__ inv, double4* __restrict__ spos) {
+                               const int* __restrict__ inv, const Mir spos) {
     const int r = gb_rep[blockIdx.x];
     const RepDesc& D = R[r];
     const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
@@ -281,7 +281,7 @@ __global__ void k_b_ghost_src(const RepDesc* __restrict__ R, const int* __restri
 __global__ void k_b_ghost_step_mirror(const RepDesc* __restrict__ R, const int* __restrict__ gb_rep,
                                       const int* __restrict__ dyn, const int* __restrict__ g_code,
                                       const int* __restrict__ g_ko, const int* __restrict__ g_kg,
-                                      double4* __restrict__ spos) {
+                                      const Mir spos) {
     const int r = gb_rep[blockIdx.x];
     const RepDesc& D = R[r];
     const int g = (blockIdx.x - D.gb0) * kBlk + threadIdx.x;
@@ -303,7 +303,7 @@ __global__ void k_b_ghost_step_mirror(const RepDesc* __restrict__ R, const int* 
 // pos of every local from its mirror entry (before a rebuild and at the end
 // of a run)
 __global__ void k_b_unmirror(const RepDesc* __restrict__ R, const int* __restrict__ eb_rep, const int* __restrict__ inv,
-                             const double4* __restrict__ spos, double4* __restrict__ pos) {
+                             const CMir spos, double4* __restrict__ pos) {
     const int r = eb_rep[blockIdx.x];
     const int i = (blockIdx.x - R[r].eb0) * kBlk + threadIdx.x;
     if (i >= R[r].n) return;
@@ -458,7 +458,7 @@ __global__ void k_b_check_work(const RepDesc* __restrict__ R, int nrep, const in
 }
 
 __global__ void k_b_mirror(const double4* __restrict__ pos, const int* __restrict__ parts, const int* __restrict__ nbinned,
-                           double4* __restrict__ spos, int cap_all) {
+                           const Mir spos, int cap_all) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= min(*nbinned, cap_all)) return;
     spos[k] = pos[parts[k]];
@@ -467,7 +467,7 @@ __global__ void k_b_mirror(const double4* __restrict__ pos, const int* __restric
 // ---------------------------------------------------------------------------
 // neighbour lists and forces: one block per tile of any replica
 __global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
-                                                          const double4* __restrict__ spos, const int* __restrict__ work_k,
+                                                          const CMir spos, const int* __restrict__ work_k,
                                                           const int* __restrict__ cell_of,
                                                           const int* __restrict__ start,
                                                           const int* __restrict__ parts, const int* __restrict__ work,
@@ -508,12 +508,12 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_nlist(const RepDesc* __re
 // force store (as k_lj_full<..., 2>); ADV 0 stores the force.
 template <int PREC, int SHIFT, int UNIT, int ADV>
 __global__ void __launch_bounds__(kMaxTileThreads) k_b_lj_full(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
-                                                               const double4* __restrict__ spos, const int* __restrict__ start,
+                                                               const CMir spos, const int* __restrict__ start,
                                                                const int* __restrict__ work, const int* __restrict__ wk,
                                                                const int* __restrict__ tile_start, const int* __restrict__ nbc,
                                                                const nb_t* __restrict__ nbl, LJArgs a,
                                                                double4* __restrict__ force, double4* __restrict__ vel, double hdt,
-                                                               double dt, double4* __restrict__ spos_next) {
+                                                               double dt, const Mir spos_next) {
     extern __shared__ double dyn[];
     const WinStage sp = win_stage(dyn, a.wcap);
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_b_lj_full(const RepDesc* __
 
 template <int SHIFT>
 __global__ void __launch_bounds__(kMaxTileThreads) k_b_energy(const RepDesc* __restrict__ R, const int* __restrict__ tile_rep,
-                                                           const double4* __restrict__ pos, const double4* __restrict__ spos,
+                                                           const double4* __restrict__ pos, const CMir spos,
                                                            const int* __restrict__ start, const int* __restrict__ parts,
                                                            const int* __restrict__ work, const int* __restrict__ tile_start,
                                                            const int* __restrict__ nbc, const nb_t* __restrict__ nbl, LJArgs a,
@@ -793,8 +793,8 @@ static void layout(Batch& B) {
     B.cell_slot.reserve(sizeof(int) * S, false, st);
     B.cell_inv.reserve(sizeof(int) * S, false, st);
     B.parts.reserve(sizeof(int) * S, false, st);
-    B.spos.reserve(sizeof(double4) * S, false, st);
-    B.spos2.reserve(sizeof(double4) * S, false, st);
+    B.spos.reserve((size_t)kMirBytes * S, false, st);
+    B.spos2.reserve((size_t)kMirBytes * S, false, st);
     B.cell_count.reserve(sizeof(int) * (size_t)B.n_cells, false, st);
     B.cell_start.reserve(sizeof(int) * ((size_t)B.n_cells + 1), false, st);
     B.tile_count.reserve(sizeof(int) * (size_t)B.n_tiles, false, st);
@@ -893,13 +893,13 @@ static void enq_rebuild(Batch& B) {
 
 static void enq_mirror(Batch& B, DevBuf& m) {
     hipLaunchKernelGGL(k_b_mirror, dim3(div_up(B.n_slots, kBlk)), dim3(kBlk), 0, B.stream, (const double4*)B.pos.as<double4>(),
-                       (const int*)B.parts.as<int>(), (const int*)(B.cell_start.as<int>() + B.n_cells), m.as<double4>(),
+                       (const int*)B.parts.as<int>(), (const int*)(B.cell_start.as<int>() + B.n_cells), Mir{m.as<double>()},
                        B.n_slots);
 }
 
 static void enq_unmirror(Batch& B, const DevBuf& m) {
     hipLaunchKernelGGL(k_b_unmirror, dim3(B.n_eb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.eb_rep.as<int>(), (const int*)B.cell_inv.as<int>(), (const double4*)m.as<double4>(),
+                       (const int*)B.eb_rep.as<int>(), (const int*)B.cell_inv.as<int>(), CMir{m.as<double>()},
                        B.pos.as<double4>());
 }
 
@@ -932,7 +932,7 @@ static void enq_nlist(Batch& B, const DevBuf& m) {
     a.cut2 = cut * cut;
     const size_t lds = l.bytes;
     hipLaunchKernelGGL(k_b_nlist, dim3(B.n_tiles), dim3(B.threads), lds, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
-                       (const int*)B.tile_rep.as<int>(), (const double4*)m.as<double4>(), (const int*)B.work_k.as<int>(),
+                       (const int*)B.tile_rep.as<int>(), CMir{m.as<double>()}, (const int*)B.work_k.as<int>(),
                        (const int*)B.cell_of.as<int>(), (const int*)B.cell_start.as<int>(),
                        (const int*)B.parts.as<int>(),
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), a, B.nb_count.as<int>(),
@@ -943,14 +943,14 @@ static void enq_nlist(Batch& B, const DevBuf& m) {
 static void enq_ghost_step_mirror(Batch& B, DevBuf& m) {
     hipLaunchKernelGGL(k_b_ghost_step_mirror, dim3(B.n_gb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.gb_rep.as<int>(), (const int*)B.dyn.as<int>(), (const int*)B.g_code.as<int>(),
-                       (const int*)B.g_ko.as<int>(), (const int*)B.g_kg.as<int>(), m.as<double4>());
+                       (const int*)B.g_ko.as<int>(), (const int*)B.g_kg.as<int>(), Mir{m.as<double>()});
 }
 
 static void enq_ghost_step(Batch& B, DevBuf& m) {
     hipLaunchKernelGGL(k_b_ghost_step, dim3(B.n_gb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.gb_rep.as<int>(), (const int*)B.dyn.as<int>(), B.pos.as<double4>(),
                        (const int*)B.g_owner.as<int>(), (const int*)B.g_code.as<int>(), (const int*)B.cell_inv.as<int>(),
-                       m.as<double4>());
+                       Mir{m.as<double>()});
 }
 
 // kick: final kick v += F dt/2 (hdt = 0 leaves v unchanged: v + f * 0 == v);
@@ -962,10 +962,10 @@ static void enq_force(Batch& B, bool kick, bool advance, const DevBuf& cur, DevB
     const double hdt = kick ? 0.5 * B.dt : 0.0;
 #define TMD_B_LJ_ADV(P, SH, U, ADV)                                                                                 \
     hipLaunchKernelGGL((k_b_lj_full<P, SH, U, ADV>), dim3(B.n_tiles), dim3(B.threads), lds, B.stream,                \
-                       (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)cur.as<double4>(), \
+                       (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), CMir{cur.as<double>()},          \
                        (const int*)B.cell_start.as<int>(), (const int*)B.work.as<int>(), (const int*)B.work_k.as<int>(), \
                        (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), (const nb_t*)B.nb_list.as<nb_t>(), \
-                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, next.as<double4>())
+                       a, B.force.as<double4>(), B.vel.as<double4>(), hdt, B.dt, Mir{next.as<double>()})
 #define TMD_B_LJ_A(P, SH, U) \
     do { if (advance) TMD_B_LJ_ADV(P, SH, U, 1); else TMD_B_LJ_ADV(P, SH, U, 0); } while (0)
 #define TMD_B_LJ_U(P, SH) \
@@ -986,7 +986,7 @@ static void enq_int1(Batch& B) {
     hipLaunchKernelGGL(k_b_int1, dim3(B.n_eb), dim3(kBlk), 0, B.stream, (const RepDesc*)B.dR.as<RepDesc>(),
                        (const int*)B.eb_rep.as<int>(), B.pos.as<double4>(), B.vel.as<double4>(),
                        (const double4*)B.force.as<double4>(), 0.5 * B.dt, B.dt, (const int*)B.cell_inv.as<int>(),
-                       B.spos.as<double4>());
+                       Mir{B.spos.as<double>()});
 }
 
 // the main.cpp step loop over `steps` steps (rebuild when (j + 1) % reneigh
@@ -1262,7 +1262,7 @@ static void batch_thermo(Batch& B, double* out) {
 #define TMD_B_EN(SH)                                                                                                 \
     hipLaunchKernelGGL(k_b_energy<SH>, dim3(B.n_tiles), dim3(B.threads), lds, B.stream,                          \
                        (const RepDesc*)B.dR.as<RepDesc>(), (const int*)B.tile_rep.as<int>(), (const double4*)B.pos.as<double4>(), \
-                       (const double4*)B.spos.as<double4>(), (const int*)B.cell_start.as<int>(), (const int*)B.parts.as<int>(), \
+                       CMir{B.spos.as<double>()}, (const int*)B.cell_start.as<int>(), (const int*)B.parts.as<int>(),        \
                        (const int*)B.work.as<int>(), (const int*)B.tile_start.as<int>(), (const int*)B.nb_count.as<int>(), \
                        (const nb_t*)B.nb_list.as<nb_t>(), a, B.epart.as<double>())
     if (B.shift) TMD_B_EN(1); else TMD_B_EN(0);
