@@ -1,5 +1,6 @@
 """Shared set-up for parity tests: the same seeded inputs go to the HIP
 library (through the C ABI) and to the CPU oracle (the checker)."""
+import itertools
 import sys
 
 import numpy as np
@@ -67,3 +68,59 @@ def force_error_atom(f, fref, floor=None):
 # the FP32-mixed bar against FP64: per-atom relative force error (floor: the
 # RMS force) at most 1e-5, both as RMS and as maximum over the atoms
 FP32_BAR = 1e-5
+
+
+def boundary_pairs(seed=5):
+    """Pairs whose squared distance, formed as the reference forms it
+    (dx*dx + dy*dy + dz*dz in FP64, no contraction), lands exactly on, one
+    ulp below or one ulp above r_nb^2 = (rc + skin)^2 (the list keeps
+    rsq <= r_nb^2, core/neighborlist.impala) and rc^2 (the force keeps
+    rsq < rc^2, core/compute.impala).  27 anchors 8 apart in a 30^3 box get
+    one partner each, placed along a seeded random direction and then moved
+    by ulp steps of its coordinates until the target is hit.  No atom lies
+    within r_nb of a face, so there are no ghosts and no periodic image in
+    range.  Returns (aabb, m, x, v, pairs), pairs = [(anchor, partner,
+    target name, rsq)]; atoms 0..26 are the anchors, 27..53 the partners."""
+    cut = RC + SKIN
+    targets = []
+    for name, t in (("cut2", cut * cut), ("rc2", RC * RC)):
+        targets += [(name + "_eq", t), (name + "_lo", np.nextafter(t, 0.0)), (name + "_hi", np.nextafter(t, 10.0))]
+    rng = np.random.default_rng(seed)
+    steps = range(-6, 7)
+    anchors, partners, pairs = [], [], []
+    for k, (a, b, c) in enumerate(itertools.product(range(3), repeat=3)):
+        anchor = np.array([6.0 + 8.0 * a, 6.0 + 8.0 * b, 6.0 + 8.0 * c])
+        name, goal = targets[k % len(targets)]
+        found = None
+        while found is None:
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            start = anchor + u * np.sqrt(goal)
+            # each coordinate moved by s ulps, s in -6..6
+            cand = []
+            for d in range(3):
+                row = {0: start[d]}
+                up = dn = start[d]
+                for s_ in range(1, 7):
+                    up = np.nextafter(up, np.inf)
+                    dn = np.nextafter(dn, -np.inf)
+                    row[s_], row[-s_] = up, dn
+                cand.append(row)
+            for sx, sy, sz in itertools.product(steps, steps, steps):
+                p = np.array([cand[0][sx], cand[1][sy], cand[2][sz]])
+                dd = anchor - p
+                if dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] == goal:
+                    found = p
+                    break
+        anchors.append(anchor)
+        partners.append(found)
+        pairs.append((k, 27 + k, name, float(goal)))
+    x = np.vstack(anchors + partners)
+    aabb = np.array([0.0, 30.0, 0.0, 30.0, 0.0, 30.0])
+    return aabb, np.ones(len(x)), x, np.zeros_like(x), pairs
+
+
+def brute_force_rsq(x):
+    """All-pairs squared distances formed as the reference forms them."""
+    d = x[:, None, :] - x[None, :, :]
+    return d[:, :, 0] * d[:, :, 0] + d[:, :, 1] * d[:, :, 1] + d[:, :, 2] * d[:, :, 2]

@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from common import FP32_BAR, RC, SKIN, announce, force_error, force_error_atom, make_pair, perturbed_fcc
+from common import (FP32_BAR, RC, SKIN, announce, boundary_pairs, brute_force_rsq, force_error, force_error_atom,
+                    make_pair, perturbed_fcc)
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +76,43 @@ def test_neighbor_lists_exact(tm, half, half_mode):
     if half and half_mode == "ref_literal":
         # the reference's literal filter keeps no ghost (index >= n)
         assert all(np.all(lo[i, :co[i]] < n) for i in range(n))
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_cutoff_boundary_pairs(tm, half):
+    """Pairs whose squared distance is exactly r_nb^2, one ulp below and one
+    ulp above, and the same around rc^2: the lists equal the oracle's entry
+    for entry (the pair at r_nb^2 kept, the one an ulp beyond dropped), and
+    the FP64 forces equal the oracle's (bit for bit with full lists), so the
+    force of a pair at rc^2 is zero and one an ulp inside is not."""
+    announce(f"cutoff boundary pairs, half={half}")
+    aabb, m, x, v, pairs = boundary_pairs()
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc, half)
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert np.array_equal(cg, co)
+    for i in range(len(cg)):
+        assert np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]), i
+    cut2 = (RC + SKIN) * (RC + SKIN)
+    for a, p, name, goal in pairs:
+        kept = p in lg[a, :cg[a]].tolist() or a in lg[p, :cg[p]].tolist()
+        assert kept == (goal <= cut2), name
+    sim.lennard_jones(half, RC)
+    orc.lj(half, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    if half:
+        assert np.allclose(fg, fo, rtol=1e-12, atol=1e-15)
+    else:
+        assert np.array_equal(fg, fo)
+    r2 = brute_force_rsq(x)
+    for a, p, name, goal in pairs:
+        if any(j not in (a, p) for j in np.nonzero(r2[a] < RC * RC)[0]):
+            continue
+        assert np.any(fg[a] != 0.0) == (goal < RC * RC), name
+    sim.close()
+    orc.close()
 
 
 @pytest.mark.parametrize("half", [False, True])

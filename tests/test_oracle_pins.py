@@ -282,6 +282,51 @@ def test_neighbor_lists_brute_force(half, mode):
         assert sorted(lst[i, :counts[i]].tolist()) == js.tolist(), i
 
 
+@pytest.mark.parametrize("half", [False, True])
+def test_list_and_force_cutoff_boundaries(half):
+    """Pairs at exactly r_nb^2, one ulp below and one ulp above (the list's
+    rsq <= r_nb^2), and the same around rc^2 (the force's rsq < rc^2): the
+    oracle's lists equal brute force, the pair at r_nb^2 is kept and the one
+    an ulp beyond is not, and the force of a pair at rc^2 (or beyond) is
+    exactly zero while one an ulp inside is not."""
+    from common import boundary_pairs, brute_force_rsq
+    aabb, m, x, v, pairs = boundary_pairs()
+    s = po.OracleSim(m, x, v, aabb, RC + SKIN)
+    s.setup(half, po.HALF_GHOST_FULL, RC + SKIN)
+    assert len(s.ghosts()["pos"]) == 0
+    counts, lst = s.neighbor_lists()
+    n = s.n
+    r2 = brute_force_rsq(x)
+    cut2 = (RC + SKIN) * (RC + SKIN)
+    for i in range(n):
+        js = np.nonzero(r2[i] <= cut2)[0]
+        js = js[js != i]
+        if half:
+            js = js[js < i]
+        assert sorted(lst[i, :counts[i]].tolist()) == js.tolist(), i
+    for a, p, name, goal in pairs:
+        kept = p in lst[a, :counts[a]].tolist() or a in lst[p, :counts[p]].tolist()
+        assert kept == (goal <= cut2), name
+    assert {name for _, _, name, _ in pairs} >= {"cut2_eq", "cut2_lo", "cut2_hi", "rc2_eq", "rc2_lo", "rc2_hi"}
+    s.lj(half, RC)
+    _, _, f = s.state()
+    rc2 = RC * RC
+    checked = set()
+    for a, p, name, goal in pairs:
+        # anchors whose only neighbour within rc is (or would be) the partner
+        others = [j for j in np.nonzero(r2[a] < rc2)[0] if j not in (a, p)]
+        if others:
+            continue
+        checked.add(name)
+        if goal < rc2:
+            assert np.any(f[a] != 0.0), name
+            assert np.array_equal(f[a], -f[p]), name  # one pair: equal and opposite
+        else:
+            assert np.array_equal(f[a], np.zeros(3)), name
+    assert checked >= {"rc2_eq", "rc2_lo", "rc2_hi"}
+    s.close()
+
+
 def test_neighbor_counts_perfect_lattice():
     s = po.lj_benchmark(8)
     counts, lst = s.neighbor_lists()
