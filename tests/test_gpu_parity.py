@@ -84,7 +84,8 @@ def test_cutoff_boundary_pairs(tm, half):
     ulp above, and the same around rc^2: the lists equal the oracle's entry
     for entry (the pair at r_nb^2 kept, the one an ulp beyond dropped), and
     the FP64 forces equal the oracle's (bit for bit with full lists), so the
-    force of a pair at rc^2 is zero and one an ulp inside is not."""
+    force of a pair at rc^2 is zero and one an ulp inside is not.  The box
+    has no ghosts; with full lists it also runs the fused step loop."""
     announce(f"cutoff boundary pairs, half={half}")
     aabb, m, x, v, pairs = boundary_pairs()
     sim, orc = make_pair(aabb, m, x, v)
@@ -111,6 +112,15 @@ def test_cutoff_boundary_pairs(tm, half):
         if any(j not in (a, p) for j in np.nonzero(r2[a] < RC * RC)[0]):
             continue
         assert np.any(fg[a] != 0.0) == (goal < RC * RC), name
+    if not half:
+        # the fused step loop on a box without ghosts (no atom within r_nb of
+        # a face): FP64 trajectory bit for bit, rebuilds included
+        assert sim.nghost == 0 and orc.ng == 0
+        sim.run_lj(45, reneigh=20, dt=0.005)
+        orc.run_lj(45, reneigh=20, dt=0.005)
+        _, xg, vg, fg = sim.state()
+        xo, vo, fo = orc.state()
+        assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
     sim.close()
     orc.close()
 
