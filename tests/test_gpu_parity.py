@@ -126,6 +126,36 @@ def test_cutoff_boundary_pairs(tm, half):
 
 
 @pytest.mark.parametrize("half", [False, True])
+def test_dense_lists_exact(tm, half):
+    """A compressed lattice (lattice constant 1.0, about 365 neighbours per
+    atom in a 6^3 box whose grid's inner cells also hold ghosts): 32-candidate
+    chunks accepted whole or nearly so, lists grown from capacity 100 past
+    700, small tiles with windows read from the mirror.  Lists equal the
+    oracle's entry for entry; FP64 forces bit for bit with full lists."""
+    announce(f"dense lattice lists, half={half}")
+    aabb, m, x, v = perturbed_fcc(6, amp=0.05, seed=7, potmin=1.0)
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc, half)
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert cg.max() > 300  # dense: chunks of 32 candidates mostly accepted
+    assert np.array_equal(cg, co)
+    for i in range(len(cg)):
+        assert np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]), i
+    sim.lennard_jones(half, RC)
+    orc.lj(half, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    if half:
+        rms, mx = force_error(fg, fo)
+        assert rms < 1e-12 and mx < 1e-11, (rms, mx)
+    else:
+        assert np.array_equal(fg, fo)
+    sim.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("half", [False, True])
 def test_list_capacity_retry(tm, half):
     """Capacity 1 grows and retries; with half lists the complementary
     columns share the capacity, and the forces come out right."""
