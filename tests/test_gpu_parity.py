@@ -125,6 +125,47 @@ def test_cutoff_boundary_pairs(tm, half):
     orc.close()
 
 
+def test_atoms_exactly_on_box_faces(tm):
+    """Atoms moved onto exactly 0 or exactly L in one coordinate (those
+    within 0.15 of a face, so the physics is unchanged): the wrap is strict
+    (core/compute.impala:118-122), so an atom at L stays at L and is binned
+    and given ghosts from there.  Lists, FP64 forces and a 45-step FP64
+    trajectory with two rebuilds equal the oracle's bit for bit."""
+    announce("atoms exactly on the box faces")
+    aabb, m, x, v = perturbed_fcc(5, amp=0.1, seed=9)
+    L = aabb[1::2] - aabb[0::2]
+    x = x.copy()
+    moved = 0
+    for i in range(len(x)):
+        d = i % 3
+        if x[i, d] < 0.15:
+            x[i, d] = 0.0 if (i // 3) % 2 else L[d]
+            moved += 1
+        elif x[i, d] > L[d] - 0.15:
+            x[i, d] = L[d] if (i // 3) % 2 else 0.0
+            moved += 1
+    assert moved > 20 and np.any(x == L[0])
+    sim, orc = make_pair(aabb, m, x, v)
+    setup_both(sim, orc)
+    cg, lg = sim.neighbor_lists()
+    co, lo = orc.neighbor_lists()
+    assert np.array_equal(cg, co)
+    for i in range(len(cg)):
+        assert np.array_equal(lg[i, :cg[i]], lo[i, :co[i]]), i
+    sim.lennard_jones(False, RC)
+    orc.lj(False, RC)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    assert np.array_equal(fg, fo)
+    sim.run_lj(45, reneigh=20, dt=0.005)
+    orc.run_lj(45, reneigh=20, dt=0.005)
+    _, xg, vg, fg = sim.state()
+    xo, vo, fo = orc.state()
+    assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(fg, fo)
+    sim.close()
+    orc.close()
+
+
 @pytest.mark.parametrize("half", [False, True])
 def test_dense_lists_exact(tm, half):
     """A compressed lattice (lattice constant 1.0, about 365 neighbours per
