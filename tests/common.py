@@ -70,7 +70,7 @@ def force_error_atom(f, fref, floor=None):
 FP32_BAR = 1e-5
 
 
-def boundary_pairs(seed=5):
+def boundary_pairs(seed=5, targets=None):
     """Pairs whose squared distance, formed as the reference forms it
     (dx*dx + dy*dy + dz*dz in FP64, no contraction), lands exactly on, one
     ulp below or one ulp above r_nb^2 = (rc + skin)^2 (the list keeps
@@ -79,12 +79,15 @@ def boundary_pairs(seed=5):
     one partner each, placed along a seeded random direction and then moved
     by ulp steps of its coordinates until the target is hit.  No atom lies
     within r_nb of a face, so there are no ghosts and no periodic image in
-    range.  Returns (aabb, m, x, v, pairs), pairs = [(anchor, partner,
-    target name, rsq)]; atoms 0..26 are the anchors, 27..53 the partners."""
-    cut = RC + SKIN
-    targets = []
-    for name, t in (("cut2", cut * cut), ("rc2", RC * RC)):
-        targets += [(name + "_eq", t), (name + "_lo", np.nextafter(t, 0.0)), (name + "_hi", np.nextafter(t, 10.0))]
+    range.  Other squared distances to hit can be given as targets, a list
+    of (name, rsq).  Returns (aabb, m, x, v, pairs), pairs = [(anchor,
+    partner, target name, rsq)]; atoms 0..26 are the anchors, 27..53 the
+    partners."""
+    if targets is None:
+        cut = RC + SKIN
+        targets = []
+        for name, t in (("cut2", cut * cut), ("rc2", RC * RC)):
+            targets += [(name + "_eq", t), (name + "_lo", np.nextafter(t, 0.0)), (name + "_hi", np.nextafter(t, 10.0))]
     rng = np.random.default_rng(seed)
     steps = range(-6, 7)
     anchors, partners, pairs = [], [], []

@@ -442,6 +442,35 @@ def test_dem_matches_oracle(tm, half):
     assert np.max(np.abs(xg - xo)) < 1e-10 and np.max(np.abs(vg - vo)) < 1e-9
 
 
+def test_dem_contact_boundary_pairs(tm):
+    """Sphere pairs (diameter 1) at exactly touching distance, one ulp
+    inside and one ulp apart, the partner moving: the dashpot term makes the
+    force of every pair in contact non-zero even at zero overlap, so the
+    contact decision (2R)^2 - rsq > 0 in FP64 is visible.  FP64 forces equal
+    the oracle's, zero exactly for the touching and the separated pairs."""
+    announce("DEM contact boundary pairs")
+    one = 1.0
+    targets = [("d2_eq", one), ("d2_lo", np.nextafter(one, 0.0)), ("d2_hi", np.nextafter(one, 2.0))]
+    aabb, m, x, v, pairs = boundary_pairs(seed=8, targets=targets)
+    v = v.copy()
+    v[27:] = [0.3, -0.2, 0.1]
+    # cutoff 1.1 > 2R: the contact test alone decides at rsq = (2R)^2
+    sim, orc = make_pair(aabb, m, x, v, cutoff=1.4, dem_radius=0.5)
+    sim.setup(True, 1.1, 0.3)
+    orc.setup(True, po.HALF_GHOST_FULL, 1.4)
+    sim.dem(True, 1.1, 0.5, 0.2, 100.0, 0.3)
+    orc.dem(True, 1.1, 0.5, 0.5, 0.2, 100.0, 0.3)
+    _, _, _, fg = sim.state()
+    _, _, fo = orc.state()
+    assert np.allclose(fg, fo, rtol=1e-12, atol=0.0)
+    for a, p, name, goal in pairs:
+        touching = goal < one  # (2R)^2 - rsq > 0
+        assert np.any(fg[a] != 0.0) == touching, name
+        assert np.any(fo[a] != 0.0) == touching, name
+    sim.close()
+    orc.close()
+
+
 def test_full_size_lattice_invariants(tm):
     """80^3 (2,048,000 atoms): size-independent invariants at t = 0."""
     announce("80^3 lattice invariants")

@@ -327,6 +327,31 @@ def test_list_and_force_cutoff_boundaries(half):
     s.close()
 
 
+def test_dem_contact_boundary():
+    """Spheres of diameter 1 at exactly touching distance, one ulp inside
+    and one ulp apart, the partner moving: only the pair one ulp inside is
+    in contact (rsq < (2R)^2); its dashpot force is non-zero and the pair's
+    forces are equal and opposite, the others exactly zero."""
+    from common import boundary_pairs
+    one = 1.0
+    targets = [("d2_eq", one), ("d2_lo", np.nextafter(one, 0.0)), ("d2_hi", np.nextafter(one, 2.0))]
+    aabb, m, x, v, pairs = boundary_pairs(seed=8, targets=targets)
+    v = v.copy()
+    v[27:] = [0.3, -0.2, 0.1]
+    # cutoff 1.1 > 2R: the contact test alone decides at rsq = (2R)^2
+    for half in (False, True):
+        s = po.OracleSim(m, x, v, aabb, 1.4)
+        s.setup(half, po.HALF_GHOST_FULL, 1.4)
+        s.dem(half, 1.1, 0.5, 0.5, 0.2, 100.0, 0.3)
+        _, _, f = s.state()
+        for a, p, name, goal in pairs:
+            if goal < one:
+                assert np.any(f[a] != 0.0) and np.array_equal(f[a], -f[p]), name
+            else:
+                assert np.array_equal(f[a], np.zeros(3)) and np.array_equal(f[p], np.zeros(3)), name
+        s.close()
+
+
 def test_neighbor_counts_perfect_lattice():
     s = po.lj_benchmark(8)
     counts, lst = s.neighbor_lists()
