@@ -22,7 +22,9 @@
 // FP64 LJ follows the reference closure operation by operation
 // (core/runtime.impala:188-201) in list order, so full-list FP64 forces are
 // bit-identical to the CPU restatement.  FP32-mixed: the distance test stays
-// FP64 (identical in/out decisions at the cutoff); the LJ algebra is FP32 and
+// FP64, on an rsq formed with FMAs (an ulp or so from the reference's
+// rounding, so an in/out decision can differ from the FP64 path's only for a
+// pair that close to rc^2); the LJ algebra is FP32 and
 // dx * f is accumulated in FP64 (each FP32 term is exact there, and the
 // extra mantissa bits absorb the summation rounding).  The Kahan variant
 // accumulates in FP32 with a compensation term instead.

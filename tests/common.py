@@ -123,6 +123,17 @@ def boundary_pairs(seed=5, targets=None):
     return aabb, np.ones(len(x)), x, np.zeros_like(x), pairs
 
 
+def fma_rsq(xi, xj):
+    """The squared distance as the FP32 force variants form it in FP64,
+    fma(dx, dx, fma(dy, dy, dz * dz)), each fused step rounded once: computed
+    exactly with rationals and rounded to the nearest double."""
+    from fractions import Fraction
+    d = xi - xj
+    dz2 = float(Fraction(d[2]) * Fraction(d[2]))
+    t = float(Fraction(d[1]) * Fraction(d[1]) + Fraction(dz2))
+    return float(Fraction(d[0]) * Fraction(d[0]) + Fraction(t))
+
+
 def brute_force_rsq(x):
     """All-pairs squared distances formed as the reference forms them."""
     d = x[:, None, :] - x[None, :, :]

@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from common import (FP32_BAR, RC, SKIN, announce, boundary_pairs, brute_force_rsq, force_error, force_error_atom,
-                    make_pair, perturbed_fcc)
+from common import (FP32_BAR, RC, SKIN, announce, boundary_pairs, brute_force_rsq, fma_rsq, force_error,
+                    force_error_atom, make_pair, perturbed_fcc)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,6 +112,28 @@ def test_cutoff_boundary_pairs(tm, half):
         if any(j not in (a, p) for j in np.nonzero(r2[a] < RC * RC)[0]):
             continue
         assert np.any(fg[a] != 0.0) == (goal < RC * RC), name
+    # the FP32 variants decide on their FMA-formed rsq, which can lie an ulp
+    # from the reference's rounding: at rc^2 their in/out decisions follow
+    # that rule exactly, and where both rules agree the forces meet the bar
+    flips = 0
+    for precision in ("fp32mixed", "fp32kahan"):
+        sim.set_precision(precision)
+        sim.lennard_jones(half, RC)
+        _, _, _, f32 = sim.state()
+        agree = np.ones(len(x), dtype=bool)
+        for a, p, name, goal in pairs:
+            if any(j not in (a, p) for j in np.nonzero(r2[a] < RC * RC)[0]):
+                continue
+            inside = fma_rsq(x[a], x[p]) < RC * RC
+            assert np.any(f32[a] != 0.0) == inside, (precision, name)
+            if inside != (goal < RC * RC):
+                agree[[a, p]] = False
+                flips += 1
+        rms, mx = force_error_atom(f32[agree], fg[agree])
+        assert rms < FP32_BAR and mx < FP32_BAR, (precision, rms, mx)
+    assert flips > 0  # the construction does hit pairs the two roundings split
+    sim.set_precision("fp64")
+    sim.lennard_jones(half, RC)  # the FP64 forces the run starts from
     if not half:
         # the fused step loop on a box without ghosts (no atom within r_nb of
         # a face): FP64 trajectory bit for bit, rebuilds included
