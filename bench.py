@@ -338,9 +338,13 @@ def dem_leg(tm, steps=1000, nc=64):
     # per-kernel timers on a separate, shorter run (they add host work)
     sim.profile(True)
     sim.profile_reset()
-    sim.run_dem(5 * REN, REN, 0.001, True, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)
+    psteps = 5 * REN
+    sim.run_dem(psteps, REN, 0.001, True, 1.0, 0.3, 0.5, 0.2, 100.0, 0.3)
     sim.barrier()
     ms, cnt = sim.profile_get("force")
+    per = {k: sim.profile_get(k)[0] for k in ("force", "integrate1", "integrate2", "ghost_sync", "mirror", "neighbor", "bin",
+                                             "borders")}
+    phases = {ph: sum(per[k] for k in ks) / psteps for ph, ks in PHASES.items()}
     sim.profile(False)
     n = sim.nlocal
     nall = n + sim.nghost
@@ -354,7 +358,7 @@ def dem_leg(tm, steps=1000, nc=64):
     sim.close()
     t = ms / max(cnt, 1)
     return {"atoms": n, "steps": steps, "seconds": t1 - t0, "atom_steps_per_s": n * steps / (t1 - t0),
-            "force_ms_avg": t, "kinetic_energy_end": ke, "half_list_entries": entries,
+            "force_ms_avg": t, "phases_ms_per_step": phases, "kinetic_energy_end": ke, "half_list_entries": entries,
             "roofline": {"bound": "hbm", "bytes_per_launch": nbytes, "achieved": nbytes / (t * 1e-3) / 1e9,
                          "time": "the force phase by HIP events: one pass computing the contact forces, both kicks and the drift",
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}}
