@@ -353,7 +353,9 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const CMir spos, 
                                                            const int* __restrict__ nbc2, const nb_t* __restrict__ nbl2,
                                                            DEMArgs a, double hdt, double dt, const Mir spos_next,
                                                            const Mir svel_next, double4* __restrict__ force,
-                                                           double4* __restrict__ vel, double4* __restrict__ pos) {
+                                                           double4* __restrict__ vel, double4* __restrict__ pos,
+                                                           const int* __restrict__ gh_start, const int* __restrict__ gh_k,
+                                                           const int* __restrict__ gh_c, double L0, double L1, double L2) {
     extern __shared__ double dyn[];
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int w0 = tile_start[tile], nloc = tile_start[tile + 1] - w0;
@@ -447,6 +449,23 @@ __global__ void __launch_bounds__(kMaxTileThreads) k_dem_fused(const CMir spos, 
                 x.z = x.z + v.z * dt;
                 spos_next[kw] = x;
                 svel_next[kw] = v;
+                // the sphere's ghosts: its new position + image shift and its
+                // new velocity (k_ghost_step_mirror_v's arithmetic)
+                if (gh_start) {
+                    const int e1 = gh_start[w + 1];
+                    for (int e = gh_start[w]; e < e1; ++e) {
+                        int ix, iy, iz;
+                        img_decode(gh_c[e], ix, iy, iz);
+                        double4 q;
+                        q.x = x.x + L0 * (double)ix;
+                        q.y = x.y + L1 * (double)iy;
+                        q.z = x.z + L2 * (double)iz;
+                        q.w = x.w;
+                        const int kg = gh_k[e];
+                        spos_next[kg] = q;
+                        svel_next[kg] = v;
+                    }
+                }
             } else {
                 const int i = work[w];
                 force[i] = make_double4(fx, fy, fz, 0.0);
@@ -470,6 +489,11 @@ void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double
     const int nb = s.tiles.ntx * s.tiles.nty * s.tiles.ntz;
     const size_t lds = (size_t)a.wcap * kDemStageBytes;
     const double hdt = 0.5 * dt;
+    // advancing: the pass also writes the ghosts' next mirror entries
+    // (from each local's ghost index), so the next step needs no ghost pass
+    if (advance && s.ng > 0 && s.ghost_mode == GHOST_FRESH && !(s.gsrc_valid && s.gh_valid)) build_ghost_index(s);
+    const bool fuse = advance && s.ng > 0 && s.ghost_mode == GHOST_FRESH && s.gsrc_valid && s.gh_valid && !has_error();
+    const int* GS = fuse ? s.gh_start.as<int>() : nullptr;
     hipEvent_t ev = nullptr;
     s.timer.begin(K_FORCE, s.stream, &ev);
 #define TMD_DEMF(P, H, A)                                                                                               \
@@ -479,7 +503,8 @@ void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double
                        (const int*)s.tile_start.as<int>(), (const int*)s.nb_count.as<int>(), (const nb_t*)s.nb_list.as<nb_t>(), \
                        (const int*)s.nb_count2.as<int>(), (const nb_t*)s.nb_list2.as<nb_t>(), a, hdt, dt,               \
                        Mir{s.spos2.as<double>()}, Mir{s.svel2.as<double>()}, s.force.as<double4>(), s.vel.as<double4>(),    \
-                       s.pos.as<double4>())
+                       s.pos.as<double4>(), GS, (const int*)s.gh_k.as<int>(), (const int*)s.gh_c.as<int>(), s.g.len[0],  \
+                       s.g.len[1], s.g.len[2])
 #define TMD_DEMF_A(P, H) do { if (advance) TMD_DEMF(P, H, 1); else TMD_DEMF(P, H, 0); } while (0)
 #define TMD_DEMF_H(P) do { if (half) TMD_DEMF_A(P, true); else TMD_DEMF_A(P, false); } while (0)
     if (s.precision == PREC_FP64) TMD_DEMF_H(PREC_FP64);
@@ -492,6 +517,7 @@ void compute_dem_fused(Sim& s, int half, double rc, double gn, double gt, double
     if (advance) {
         std::swap(s.spos, s.spos2);
         std::swap(s.svel, s.svel2);
+        s.dem_ghosts_done = fuse || s.ng == 0;
     } else {
         // the locals are back in atom order; their ghosts follow from them
         s.dem_mirrored = 0;

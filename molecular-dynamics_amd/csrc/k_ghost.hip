@@ -25,12 +25,6 @@ struct BorderArgs {
     double len0, len1, len2;
 };
 
-__device__ __forceinline__ int img_encode(int ix, int iy, int iz) { return (ix + 1) + 3 * (iy + 1) + 9 * (iz + 1); }
-__device__ __forceinline__ void img_decode(int c, int& ix, int& iy, int& iz) {
-    ix = c % 3 - 1;
-    iy = (c / 3) % 3 - 1;
-    iz = c / 9 - 1;
-}
 
 __device__ __forceinline__ double comp(const double4& p, int d) { return d == 0 ? p.x : (d == 1 ? p.y : p.z); }
 
@@ -272,8 +266,64 @@ __global__ void k_ghost_src(const int* __restrict__ g_owner, const int* __restri
     g_kg[g] = inv[n + g];
 }
 
+// Each local's ghosts by the local's work slot (a CSR over work slots), so
+// that a force pass that advances a local can also write its ghosts' mirror
+// entries.  Within one local the order of its ghosts is that of the atomics
+// (each ghost's entry is written independently of the others).
+__global__ void k_inv_work(const int* __restrict__ work, int n, int* __restrict__ inv_work) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < n) inv_work[work[w]] = w;
+}
+
+__global__ void k_gh_count(const int* __restrict__ g_owner, const int* __restrict__ g_kg, int ng,
+                           const int* __restrict__ inv_work, int* __restrict__ cnt) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng || g_kg[g] < 0) return;
+    atomicAdd(&cnt[inv_work[g_owner[g]]], 1);
+}
+
+__global__ void k_gh_fill(const int* __restrict__ g_owner, const int* __restrict__ g_kg, const int* __restrict__ g_code,
+                          int ng, const int* __restrict__ inv_work, const int* __restrict__ start, int* __restrict__ fill,
+                          int* __restrict__ gh_k, int* __restrict__ gh_c) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng || g_kg[g] < 0) return;
+    const int w = inv_work[g_owner[g]];
+    const int e = start[w] + atomicAdd(&fill[w], 1);
+    gh_k[e] = g_kg[g];
+    gh_c[e] = g_code[g] & 0xff;
+}
+
+void build_ghost_index(Sim& s) {
+    s.gh_valid = 0;
+    if (has_error() || s.n == 0) return;
+    if (!s.gsrc_valid) build_ghost_src(s);
+    const int n = s.n, ng = s.ng;
+    s.gh_start.reserve(sizeof(int) * ((size_t)n + 1), false, s.stream);
+    s.gh_aux.reserve(sizeof(int) * 2 * (size_t)n, false, s.stream);
+    s.gh_k.reserve(sizeof(int) * (size_t)std::max(ng, 1), false, s.stream);
+    s.gh_c.reserve(sizeof(int) * (size_t)std::max(ng, 1), false, s.stream);
+    if (has_error()) return;
+    int* inv_work = s.gh_aux.as<int>();
+    int* cnt = inv_work + n;
+    const int nt = 256;
+    hipLaunchKernelGGL(k_inv_work, dim3(div_up(n, nt)), dim3(nt), 0, s.stream, (const int*)s.work.as<int>(), n, inv_work);
+    TMD_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)n, s.stream));
+    if (ng > 0)
+        hipLaunchKernelGGL(k_gh_count, dim3(div_up(ng, nt)), dim3(nt), 0, s.stream, (const int*)s.g_owner.as<int>(),
+                           (const int*)s.g_kg.as<int>(), ng, (const int*)inv_work, cnt);
+    exclusive_scan_int(cnt, s.gh_start.as<int>(), n, s.gh_start.as<int>() + n, s);
+    TMD_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)n, s.stream));
+    if (ng > 0)
+        hipLaunchKernelGGL(k_gh_fill, dim3(div_up(ng, nt)), dim3(nt), 0, s.stream, (const int*)s.g_owner.as<int>(),
+                           (const int*)s.g_kg.as<int>(), (const int*)s.g_code.as<int>(), ng, (const int*)inv_work,
+                           (const int*)s.gh_start.as<int>(), cnt, s.gh_k.as<int>(), s.gh_c.as<int>());
+    TMD_HIP(hipGetLastError());
+    if (!has_error()) s.gh_valid = 1;
+}
+
 void build_ghost_src(Sim& s) {
     s.gsrc_valid = 0;
+    s.gh_valid = 0;
     if (has_error() || s.ng == 0) return;
     s.g_ko.reserve(sizeof(int) * (size_t)s.cap_all, false, s.stream);
     s.g_kg.reserve(sizeof(int) * (size_t)s.cap_all, false, s.stream);

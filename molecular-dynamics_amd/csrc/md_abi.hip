@@ -663,7 +663,7 @@ void md_deallocate_grid() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     s.timer.collect();
     DevBuf* bufs[] = {&s.pos, &s.vel, &s.force, &s.mass, &s.sort_buf, &s.g_owner, &s.g_src, &s.g_code, &s.cell_of, &s.cell_slot,
-                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.g_ko, &s.g_kg, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel, &s.svel2,
+                      &s.cell_count, &s.cell_start, &s.cell_parts, &s.cell_inv, &s.work_start, &s.work, &s.work_k, &s.work_c, &s.g_ko, &s.g_kg, &s.gh_start, &s.gh_k, &s.gh_c, &s.gh_aux, &s.tile_start, &s.spos, &s.spos2, &s.face, &s.svel, &s.svel2,
                       &s.nb_count, &s.nb_list, &s.nb_count2, &s.nb_list2,
                       &s.scan_blocks, &s.scan_aux, &s.red, &s.gbuf, &s.adj_x0, &s.adj_v0, &s.adj_ckx,
                       &s.adj_ckv, &s.adj_segx, &s.adj_segv, &s.adj_xbar, &s.adj_vbar, &s.adj_fbar, &s.adj_part,
@@ -1130,9 +1130,13 @@ static int run_loop(int steps, int reneigh_every, double dt, bool half, double c
                 }
                 rebuild(s, hf, cutoff + skin);
             } else if (s.dem_mirrored) {
-                s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
-                dem_ghost_step(s);
-                s.timer.end(K_GHOST_SYNC, s.stream, ev);
+                // the previous step pass wrote the ghosts too, unless it
+                // could not (no ghost index)
+                if (!s.dem_ghosts_done) {
+                    s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
+                    dem_ghost_step(s);
+                    s.timer.end(K_GHOST_SYNC, s.stream, ev);
+                }
             } else {
                 s.timer.begin(K_GHOST_SYNC, s.stream, &ev);
                 refresh_ghosts(s);

@@ -111,6 +111,8 @@ struct Sim {
     int vel_mirrored = 0;   // local velocities live in svel, in mirror order (fused step loop)
     int vel_mirror_wanted = 0;  // fused step loop: refresh_mirror also moves the velocities to svel
     int gsrc_valid = 0;     // g_ko / g_kg match the ghost table and the binning
+    int gh_valid = 0;       // gh_start / gh_k / gh_c match the ghost table, binning and work order
+    int dem_ghosts_done = 0;  // DEM step loop: the last force pass also wrote the ghosts' mirror entries
     int defer_window_sync = 0;  // rebuild in progress: build_windows may leave its statistics pending
     int windows_pending = 0;    // window statistics queued into h_winstats, not yet applied (finish_windows)
     int* h_winstats = nullptr;  // pinned host copy of the window statistics (4 ints)
@@ -150,6 +152,8 @@ struct Sim {
     int dem_mirrored = 0;              // DEM step loop: positions and velocities (ghosts too) live in the mirrors
     DevBuf face;                       // per atom: border selection bits of the three ghost passes (u8)
     DevBuf g_ko, g_kg;                 // per ghost: mirror index of its owner and its own (-1: outside the grid)
+    DevBuf gh_start, gh_k, gh_c;       // ghosts by owner work slot: CSR start [n + 1], mirror index, image code
+    DevBuf gh_aux;                     // per work slot: owner's work slot scratch, counts and fill cursors
     DevBuf nb_count, nb_list;
     DevBuf nb_count2, nb_list2;        // half lists: the complementary columns (local j > i)
     DevBuf scan_blocks, scan_aux, counters, red, gbuf;
@@ -196,6 +200,7 @@ void refresh_ghosts(Sim& s);
 // one pass (after positions were advanced inside the force pass)
 void step_refresh(Sim& s);
 void build_ghost_src(Sim& s);  // g_ko / g_kg (binning, after the CSR is built)
+void build_ghost_index(Sim& s);  // each local's ghosts by work slot (gh_start / gh_k / gh_c)
 void sync_pos_from_mirror(Sim& s);
 void mirror_velocities(Sim& s, bool into_mirror);
 
@@ -255,6 +260,14 @@ struct PbcArgs {
     double lo[3], hi[3], len[3];
 };
 PbcArgs pbc_args(const Sim& s);
+// image shift (ix, iy, iz) in {-1, 0, 1}^3 as one code 0..26
+__device__ __forceinline__ int img_encode(int ix, int iy, int iz) { return (ix + 1) + 3 * (iy + 1) + 9 * (iz + 1); }
+__device__ __forceinline__ void img_decode(int c, int& ix, int& iy, int& iz) {
+    ix = c % 3 - 1;
+    iy = (c / 3) % 3 - 1;
+    iz = c / 9 - 1;
+}
+
 __host__ __device__ inline void pbc_wrap(double4& p, const PbcArgs& a) {
     double q;
     q = p.x; if (p.x < a.lo[0]) q += a.len[0]; if (p.x > a.hi[0]) q -= a.len[0]; p.x = q;
