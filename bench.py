@@ -352,8 +352,8 @@ def dem_leg(tm, steps=1000, nc=64):
     entries = int(counts.astype(np.int64).sum())
     # the step pass in half-list form: counts, 4-byte entries, FP64 positions
     # and velocities of locals and ghosts read once, the advanced positions
-    # and velocities of the locals written once
-    nbytes = 4 * n + 4 * entries + 48 * nall + 48 * n
+    # and velocities of the locals and of their ghosts written once
+    nbytes = 4 * n + 4 * entries + 48 * nall + 48 * nall
     ke = sim.thermo(1.0)["KE"]
     sim.close()
     t = ms / max(cnt, 1)
